@@ -1,0 +1,164 @@
+/*
+ * libmfdfd — MI355X (gfx950) implementation of the MaxwellFDFD frequency-domain
+ * hot path behind a plain C ABI: Yee-grid curl and material-operator assembly
+ * with PML-stretched ∆l, material assignment with subpixel smoothing, the
+ * matrix-free curl-curl operator, and the iterative solve of
+ *     (Cₘ μ⁻¹ Cₑ − ω² ε) e = −iω jₑ − Cₘ μ⁻¹ jₘ.
+ *
+ * Conventions follow the reference package (EMinsight/MaxwellFDM.jl, module
+ * MaxwellFDFD, src/model/model.jl:32-81):
+ *   - complex values are two interleaved doubles (Julia ComplexF64);
+ *   - CSC outputs are Julia SparseMatrixCSC{ComplexF64,Int64}: 1-based colptr
+ *     (length n+1) and rowval (length nnz), rows ascending within a column;
+ *   - parameter arrays εarr/μarr are (Nx,Ny,Nz,3,3) column-major, field arrays
+ *     (Nx,Ny,Nz,3) column-major; DOF vectors are component-first
+ *     (order_cmpfirst) or component-last;
+ *   - every function returns MFD_OK or an error code; mfd_last_error() gives
+ *     a message for the calling thread.  Nothing keeps caller pointers after
+ *     returning and nothing calls back into the caller.
+ *   - host pointers unless the name ends in _dev (device pointers on the
+ *     context's GPU, in the context's HIP stream order after a full sync).
+ */
+#ifndef MFDFD_H
+#define MFDFD_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MFD_OK = 0,
+    MFD_ERR_ARG = 1,          /* Julia ArgumentError analogue (source.jl:214,230) */
+    MFD_ERR_UNSUPPORTED = 2,  /* model.jl:241-242 (ft not EE/HH), :236 (Pμ not diagonal) */
+    MFD_ERR_HIP = 3,
+    MFD_ERR_NOCONV = 4,
+    MFD_ERR_ALLOC = 5
+};
+enum { MFD_EE = 0, MFD_HH = 1 };      /* FieldType */
+enum { MFD_PRIM = 0, MFD_DUAL = 1 };  /* GridType */
+enum { MFD_FP64 = 0, MFD_MIXED = 1 }; /* solve precision */
+
+typedef struct { double re, im; } mfd_c64;
+
+/* Grid + Model boundary settings (MaxwellBase Grid(lprim, isbloch) and the
+ * Model fields boundft/Npml/kbloch/ωpml/order_cmpfirst, model.jl:37-72). */
+typedef struct {
+    int64_t N[3];
+    const double *lprim[3];   /* N[w]+1 primal plane locations, both bounds */
+    int32_t isbloch[3];
+    int32_t boundft[3];       /* MFD_EE / MFD_HH per axis */
+    int64_t npml[2][3];       /* Npml[negative/positive end][axis] */
+    mfd_c64 omega_pml;        /* ωpml (set_ωpml!, model.jl:86) */
+    double kbloch[3];         /* set_kbloch!, model.jl:89 */
+    double pml_m;             /* SC-PML polynomial degree (4) */
+    double pml_lnR;           /* SC-PML ln(target reflection) (-16) */
+    int32_t order_cmpfirst;   /* model.jl:72 */
+} mfd_grid;
+
+/* Shapes of add_obj! (model.jl:107-120): kind 0 = ball (c, r), 1 = cuboid (c, s). */
+typedef struct {
+    int32_t kind;
+    int32_t pad;
+    double c[3];
+    double r;
+    double s[3];
+} mfd_shape;
+
+typedef struct mfd_ctx mfd_ctx;
+typedef struct mfd_op mfd_op;
+
+typedef struct {
+    double tol;               /* relative residual ||b-Ae||/||b|| in FP64 (1e-8) */
+    int64_t maxit;            /* total Krylov iterations allowed per RHS */
+    int32_t precision;        /* MFD_FP64 or MFD_MIXED (FP32 inner, FP64 refinement) */
+    int32_t jacobi;           /* 1 = Jacobi diagonal scaling (default), 0 = none */
+    double inner_tol;         /* mixed: inner relative reduction per cycle (1e-4) */
+    int32_t check_every;      /* iterations between host convergence checks (32) */
+    int32_t use_graph;        /* capture a chunk of iterations in a hipGraph */
+} mfd_solve_opts;
+
+typedef struct {
+    int64_t iterations;       /* Krylov iterations (max over RHS) */
+    int64_t outer_cycles;     /* refinement cycles (mixed) */
+    double relres;            /* max over RHS of the final FP64 relative residual */
+    double seconds;           /* device time of the solve */
+    int32_t converged;
+} mfd_solve_stats;
+
+const char *mfd_version(void);
+const char *mfd_last_error(void);
+
+/* One GPU, one HIP stream.  Not thread-safe: the caller serialises. */
+int mfd_ctx_create(int device, mfd_ctx **out);
+int mfd_ctx_destroy(mfd_ctx *ctx);
+
+/* Stretched ∆l tables, replaces MaxwellBase create_stretched_∆l called at
+ * model.jl:126.  out: [gt = PRIM, DUAL][axis 0..2][N[axis]] complex. */
+int mfd_stretched_dl(mfd_ctx *ctx, const mfd_grid *g, mfd_c64 *out);
+
+/* distweights (source.jl:153-154, 208-306): 1-based indices as in Julia.
+ * Returns MFD_ERR_ARG where the reference throws ArgumentError. */
+int mfd_distweights(double c, int gt, const double bounds[2], const double *l, const double *dl,
+                    int64_t N, int isbloch, int64_t ind[2], double wt[2]);
+
+/* add_srce!/add_srcm! with a PointSrc (model.jl:184-200, pointsrc.jl:64-103).
+ * p is normalised here as PointSrc's constructor does (pointsrc.jl:58).
+ * jarr: (Nx,Ny,Nz,3) complex, accumulated into. */
+int mfd_add_pointsrc(const mfd_grid *g, int ft, const double c[3], const double p[3], mfd_c64 Idr,
+                     mfd_c64 *jarr);
+
+/* create_curl (model.jl:171-172): which 0 = Cₑ (E -> H), 1 = Cₘ (H -> E).
+ * Two phases: query nnz, then fill caller-allocated buffers. */
+int mfd_curl_nnz(mfd_ctx *ctx, const mfd_grid *g, int which, int64_t *nnz);
+int mfd_curl_csc(mfd_ctx *ctx, const mfd_grid *g, int which, int64_t *colptr, int64_t *rowval,
+                 mfd_c64 *nzval);
+
+/* calc_matparams! (full.jl:16-70): assignment + Kottke subpixel smoothing.
+ * eps_t/mu_t: nobj 3x3 column-major tensors; objects later in the list win. */
+int mfd_matparams(mfd_ctx *ctx, const mfd_grid *g, int64_t nobj, const mfd_shape *shapes,
+                  const mfd_c64 *eps_t, const mfd_c64 *mu_t, mfd_c64 *eps_arr, mfd_c64 *mu_arr);
+
+/* create_paramop (model.jl:152-155): which 0 = Pε, 1 = Pμ. */
+int mfd_paramop_nnz(mfd_ctx *ctx, const mfd_grid *g, int which, const mfd_c64 *parr, int64_t *nnz);
+int mfd_paramop_csc(mfd_ctx *ctx, const mfd_grid *g, int which, const mfd_c64 *parr,
+                    int64_t *colptr, int64_t *rowval, mfd_c64 *nzval);
+
+/* create_A (model.jl:225-246), A = Cₘ (Pμ \ Cₑ) − ω² Pε for ft = EE. */
+int mfd_A_nnz(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr,
+              const mfd_c64 *mu_arr, int64_t *nnz);
+int mfd_A_csc(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr,
+              const mfd_c64 *mu_arr, int64_t *colptr, int64_t *rowval, mfd_c64 *nzval);
+
+/* create_b (model.jl:251-274) for ft = EE: b = −Cₘ (Pμ \ jₘ) − iω jₑ. */
+int mfd_create_b(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *mu_arr,
+                 const mfd_c64 *je, const mfd_c64 *jm, mfd_c64 *b);
+
+/* Matrix-free operator for the A of create_A (model.jl:235-237), device
+ * resident.  eps_arr/mu_arr as produced by create_paramops (host). */
+int mfd_op_create(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr,
+                  const mfd_c64 *mu_arr, mfd_op **out);
+int mfd_op_destroy(mfd_op *op);
+/* y = A x for nrhs vectors stored one after another (host pointers). */
+int mfd_op_apply(mfd_op *op, const mfd_c64 *x, mfd_c64 *y, int64_t nrhs);
+/* device pointers; prec MFD_FP64 (complex double) or MFD_MIXED (complex float) */
+int mfd_op_apply_dev(mfd_op *op, const void *x, void *y, int64_t nrhs, int prec);
+/* Average device time (ms) of the last n applies of the dominant stencil kernel */
+int mfd_op_bench_apply(mfd_op *op, int64_t nrhs, int prec, int iters, double *ms_per_apply);
+
+/* Solve A e = b (the user's `A \ b`); e holds the initial guess on entry. */
+void mfd_solve_opts_default(mfd_solve_opts *o);
+int mfd_solve(mfd_op *op, const mfd_c64 *b, mfd_c64 *e, int64_t nrhs, const mfd_solve_opts *o,
+              mfd_solve_stats *st);
+int mfd_solve_dev(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_solve_opts *o,
+                  mfd_solve_stats *st);
+/* Adjoint solve Aᵀ λ = g using Aᵀ = S_E A S_E⁻¹ (complex symmetry). */
+int mfd_solve_adjoint(mfd_op *op, const mfd_c64 *g, mfd_c64 *lam, const mfd_solve_opts *o,
+                      mfd_solve_stats *st);
+/* ∂f/∂ε per E DOF for real f with Aᵀλ = (∂f/∂e)ᵀ: grad = 2ω² Re(λ ∘ e)
+ * (derivative through the diagonal of −ω²Pε). */
+int mfd_adjoint_grad(mfd_op *op, const mfd_c64 *e, const mfd_c64 *lam, double *grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1019 @@
+// Matrix-free curl-curl operator and device-resident symmetric QMR solver.
+//
+// A = Cₘ μ⁻¹ Cₑ − ω² Pε (create_A, reference src/model/model.jl:235-237) is
+// applied without forming the CSC.  The solver (the user's `A \ b`, which the
+// reference leaves to Julia) works on the complex-symmetric
+//     P = Q T A T⁻¹ Q,   T = S_E^{1/2} (separable), Q = diag(A)^{-1/2},
+// where S_E is the stretched E-cell volume; Aᵀ = S_E A S_E⁻¹ makes P = Pᵀ.
+// P needs only Q per unknown: its diagonal mass term −ω² ε q² equals
+// 1 − c q² with c the (separable) curl-curl diagonal, so the operator reads
+// the iterate, Q and (when ε is anisotropic) the corner ε — never ε itself.
+//
+// Boundary rules (all boundft = EE): Cₑ forward differences with the ghost at
+// the positive end, Cₘ backward differences with the ghost at the negative
+// end; a Bloch ghost is the wrapped value times e^{∓ikL}, a non-Bloch ghost
+// is zero.
+#include "op.hpp"
+#include "assembly.hpp"
+#include <cmath>
+#include <cstring>
+
+namespace mfd {
+
+constexpr int NPART = 2048;   // max blocks per reduction (per RHS)
+constexpr int BS = 256;
+enum { MODE_A = 0, MODE_P = 1 };
+enum { EPI_STORE = 0, EPI_RESID = 1, EPI_DOT = 2 };
+
+// --------------------------------------------------------------- helpers
+
+template <typename R>
+__device__ inline cx<R> czero() { return mk<R>((R)0, (R)0); }
+template <typename R>
+__device__ inline cx<R> cone() { return mk<R>((R)1, (R)0); }
+
+template <typename R>
+__device__ inline void cell3(const DGrid<R> &g, int64_t p, int &i, int &j, int &k) {
+    i = (int)(p % g.N[0]);
+    int64_t t = p / g.N[0];
+    j = (int)(t % g.N[1]);
+    k = (int)(t / g.N[1]);
+}
+template <typename R>
+__device__ inline void dof2cp(const DGrid<R> &g, int64_t e, int &c, int64_t &p) {
+    if (g.cmpfirst) { c = (int)(e % 3); p = e / 3; }
+    else { c = (int)(e / g.M); p = e - (int64_t)c * g.M; }
+}
+// T (or T⁻¹) for E_c at (i,j,k): product over axes of sqrt(s∆l) at E_c's grid type
+template <typename R>
+__device__ inline cx<R> tfac(const DGrid<R> &g, int c, int i, int j, int k, bool inv) {
+    const cx<R> *const(*t)[3] = inv ? g.ssqinv : g.ssq;
+    cx<R> a = t[c == 0 ? DUAL : PRIM][0][i];
+    cx<R> b = t[c == 1 ? DUAL : PRIM][1][j];
+    cx<R> d = t[c == 2 ? DUAL : PRIM][2][k];
+    return (a * b) * d;
+}
+
+// scaled iterate value at E_c(i,j,k); indices may be -1 or N on one or more
+// axes (ghosts): Bloch wraps with the phase, non-Bloch gives zero
+template <typename R, int MODE>
+__device__ inline cx<R> ldX(const DGrid<R> &g, const OpData<R> &od, const cx<R> *__restrict__ x, int c, int i,
+                            int j, int k) {
+    cx<R> ph = cone<R>();
+    bool wr = false;
+    int id[3] = {i, j, k};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (id[a] >= g.N[a]) {
+            if (!g.bloch[a]) return czero<R>();
+            id[a] -= g.N[a]; ph = ph * g.ph[a]; wr = true;
+        } else if (id[a] < 0) {
+            if (!g.bloch[a]) return czero<R>();
+            id[a] += g.N[a]; ph = ph * g.phinv[a]; wr = true;
+        }
+    }
+    int64_t p = id[0] + (int64_t)g.N[0] * (id[1] + (int64_t)g.N[1] * id[2]);
+    cx<R> v = x[dofidx(g, c, p)];
+    if (MODE == MODE_P) v = v * (od.q[3 * p + c] * tfac(g, c, id[0], id[1], id[2], true));
+    return wr ? v * ph : v;
+}
+
+// H_u at (i,j,k) = ∂_v X_w − ∂_w X_v (forward, v = u+1, w = u+2), times 1/μ_u;
+// one index may be -1 (backward ghost of Cₘ)
+template <typename R, int MODE, bool MU>
+__device__ inline cx<R> Hval(const DGrid<R> &g, const OpData<R> &od, const cx<R> *__restrict__ x, int u, int i,
+                             int j, int k) {
+    cx<R> ph = cone<R>();
+    bool wr = false;
+    int id[3] = {i, j, k};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (id[a] < 0) {
+            if (!g.bloch[a]) return czero<R>();
+            id[a] += g.N[a]; ph = ph * g.phinv[a]; wr = true;
+        }
+    int v = u == 0 ? 1 : (u == 1 ? 2 : 0), w = u == 0 ? 2 : (u == 1 ? 0 : 1);
+    int iv[3] = {id[0], id[1], id[2]}, iw[3] = {id[0], id[1], id[2]};
+    iv[v] += 1;
+    iw[w] += 1;
+    cx<R> x0w = ldX<R, MODE>(g, od, x, w, id[0], id[1], id[2]);
+    cx<R> x1w = ldX<R, MODE>(g, od, x, w, iv[0], iv[1], iv[2]);
+    cx<R> x0v = ldX<R, MODE>(g, od, x, v, id[0], id[1], id[2]);
+    cx<R> x1v = ldX<R, MODE>(g, od, x, v, iw[0], iw[1], iw[2]);
+    cx<R> h = g.sdlinv[DUAL][v][id[v]] * (x1w - x0w) - g.sdlinv[DUAL][w][id[w]] * (x1v - x0v);
+    if (MU) {
+        int64_t p = id[0] + (int64_t)g.N[0] * (id[1] + (int64_t)g.N[1] * id[2]);
+        h = h * od.muinv[3 * p + u];
+    }
+    return wr ? h * ph : h;
+}
+
+// curl-curl diagonal for E_u at (i,j,k): Σ over the two derivative axes
+template <typename R, bool MU>
+__device__ inline cx<R> ccdiag(const DGrid<R> &g, const OpData<R> &od, int u, int i, int j, int k) {
+    int id[3] = {i, j, k};
+    int64_t p = i + (int64_t)g.N[0] * (j + (int64_t)g.N[1] * k);
+    cx<R> s = czero<R>();
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        int a = t == 0 ? (u == 0 ? 1 : (u == 1 ? 2 : 0)) : (u == 0 ? 2 : (u == 1 ? 0 : 1));
+        int hc = t == 0 ? (u == 0 ? 2 : (u == 1 ? 0 : 1)) : (u == 0 ? 1 : (u == 1 ? 2 : 0));  // H component
+        int ia = id[a], Na = g.N[a];
+        cx<R> m0 = cone<R>(), m1 = cone<R>();
+        int64_t sta = a == 0 ? 1 : (a == 1 ? (int64_t)g.N[0] : (int64_t)g.N[0] * g.N[1]);
+        if (MU) {
+            m0 = od.muinv[3 * p + hc];
+            int64_t pm = ia > 0 ? p - sta : p + (int64_t)(Na - 1) * sta;
+            m1 = od.muinv[3 * pm + hc];
+        }
+        cx<R> ie = g.sdlinv[PRIM][a][ia];
+        cx<R> term;
+        if (Na == 1 && g.bloch[a]) {
+            cx<R> f = mk<R>((R)2, (R)0) - g.ph[a] - g.phinv[a];
+            term = (ie * (g.sdlinv[DUAL][a][0] * m0)) * f;
+        } else if (ia == 0 && !g.bloch[a]) {
+            term = ie * (g.sdlinv[DUAL][a][0] * m0);
+        } else {
+            int im = ia > 0 ? ia - 1 : Na - 1;
+            term = ie * (g.sdlinv[DUAL][a][ia] * m0 + g.sdlinv[DUAL][a][im] * m1);
+        }
+        s = s + term;
+    }
+    return s;
+}
+
+// block reduction of NV doubles; result valid in thread 0
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV]) {
+    __shared__ double sh[BS / 64][NV];
+#pragma unroll
+    for (int a = 0; a < NV; ++a)
+        for (int o = 32; o > 0; o >>= 1) v[a] += __shfl_down(v[a], o, 64);
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int a = 0; a < NV; ++a) sh[wv][a] = v[a];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int a = 0; a < NV; ++a) {
+            double s = 0;
+            for (int b = 0; b < BS / 64; ++b) s += sh[b][a];
+            v[a] = s;
+        }
+    __syncthreads();
+}
+
+// write this block's partial; returns true in every thread of the last block,
+// which then holds the fixed-order total in tot (thread 0)
+template <int NV>
+__device__ inline bool reduce_last(double (&v)[NV], double *part, unsigned *cnt, double (&tot)[NV]) {
+    __shared__ int s_last;
+    block_sum<NV>(v);
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < NV; ++a) part[4 * blockIdx.x + a] = v[a];
+        __threadfence();
+        unsigned prev = atomicAdd(cnt, 1u);
+        s_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __threadfence();
+    double acc[NV];
+    for (int a = 0; a < NV; ++a) acc[a] = 0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += BS)
+        for (int a = 0; a < NV; ++a) acc[a] += __hip_atomic_load(&part[4 * b + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    block_sum<NV>(acc);
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < NV; ++a) tot[a] = acc[a];
+        *cnt = 0;
+    }
+    return true;
+}
+
+// ------------------------------------------------------------ the stencil
+
+// y = A x (MODE_A) or y = P x (MODE_P) for the RHS blockIdx.y.
+// EPI_STORE: y = op(x).  EPI_RESID: y = b − op(x), reduce ||y||² into rn.
+// EPI_DOT: y = op(x), reduce xᵀy into the solver scalars (σ, α).
+template <typename R, int MODE, bool OFFD, bool MU, int EPI>
+__global__ __launch_bounds__(BS) void k_apply(DGrid<R> g, OpData<R> od, const cx<R> *__restrict__ xall,
+                                              cx<R> *__restrict__ yall, const c64 *__restrict__ ball, int64_t vs,
+                                              double *part, unsigned *cnt, Scal *scal, double *rn) {
+    const int rhs = blockIdx.y;
+    const cx<R> *x = xall + rhs * vs;
+    cx<R> *y = yall + rhs * vs;
+    bool skip = false;
+    if (EPI == EPI_DOT) {
+        if (scal[rhs].done) return;
+        skip = scal[rhs].conv != 0;
+    }
+    double acc[2] = {0.0, 0.0};
+    if (!skip) {
+        for (int64_t p = blockIdx.x * (int64_t)BS + threadIdx.x; p < g.M; p += (int64_t)gridDim.x * BS) {
+            int i, j, k;
+            cell3(g, p, i, j, k);
+            cx<R> H[3], Hm[3][3];   // H_c(p); Hm[c][a] = H_c(p - â)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) H[c] = Hval<R, MODE, MU>(g, od, x, c, i, j, k);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    if (a == c) continue;
+                    int id[3] = {i, j, k};
+                    id[a] -= 1;
+                    Hm[c][a] = Hval<R, MODE, MU>(g, od, x, c, id[0], id[1], id[2]);
+                }
+            int id[3] = {i, j, k};
+#pragma unroll
+            for (int u = 0; u < 3; ++u) {
+                int v = u == 0 ? 1 : (u == 1 ? 2 : 0), w = u == 0 ? 2 : (u == 1 ? 0 : 1);
+                // E_u = ∂_v H_w − ∂_w H_v (backward)
+                cx<R> cc = g.sdlinv[PRIM][v][id[v]] * (H[w] - Hm[w][v]) - g.sdlinv[PRIM][w][id[w]] * (H[v] - Hm[v][w]);
+                int64_t e = dofidx(g, u, p);
+                cx<R> xu = x[e];
+                cx<R> out;
+                cx<R> lf;
+                if (MODE == MODE_A) {
+                    out = cc - od.om2 * (od.epsd[3 * p + u] * xu);
+                } else {
+                    cx<R> qu = od.q[3 * p + u];
+                    lf = qu * tfac(g, u, i, j, k, false);
+                    cx<R> cd = ccdiag<R, MU>(g, od, u, i, j, k);
+                    out = lf * cc + (cone<R>() - cd * (qu * qu)) * xu;
+                }
+                if (OFFD) {
+                    cx<R> term = czero<R>();
+#pragma unroll
+                    for (int t = 0; t < 2; ++t) {
+                        int ww = t == 0 ? v : w;   // input component
+                        int eix = (u + ww == 1) ? 0 : ((u + ww == 2) ? 1 : 2);   // (xy, xz, yz)
+#pragma unroll
+                        for (int oc = 0; oc < 2; ++oc) {
+                            int cid[3] = {i, j, k};
+                            cx<R> fo = mk<R>((R)0.5, (R)0);
+                            if (oc == 1) {
+                                cid[u] += 1;
+                                if (cid[u] >= g.N[u]) {
+                                    if (!g.bloch[u]) continue;
+                                    cid[u] -= g.N[u];
+                                    fo = (R)0.5 * g.ph[u];
+                                }
+                            }
+                            int64_t cp = cid[0] + (int64_t)g.N[0] * (cid[1] + (int64_t)g.N[1] * cid[2]);
+                            cx<R> ec = od.epso[3 * cp + eix];
+                            int cw = cid[ww];
+                            cx<R> ia = g.sdlinv[PRIM][ww][cw];
+                            cx<R> wi0 = ((R)0.5 * g.sdl[DUAL][ww][cw]) * ia;
+                            cx<R> s = wi0 * ldX<R, MODE>(g, od, x, ww, cid[0], cid[1], cid[2]);
+                            int fid[3] = {cid[0], cid[1], cid[2]};
+                            fid[ww] -= 1;
+                            if (fid[ww] >= 0 || g.bloch[ww]) {
+                                int fi = fid[ww] >= 0 ? fid[ww] : g.N[ww] - 1;
+                                cx<R> wi1 = ((R)0.5 * g.sdl[DUAL][ww][fi]) * ia;
+                                s = s + wi1 * ldX<R, MODE>(g, od, x, ww, fid[0], fid[1], fid[2]);
+                            }
+                            term = term + (fo * ec) * s;
+                        }
+                    }
+                    if (MODE == MODE_A) out = out - od.om2 * term;
+                    else out = out - od.om2 * (lf * term);
+                }
+                if (EPI == EPI_RESID) {
+                    c64 bb = ball[rhs * vs + e];
+                    cx<R> rr = mk<R>((R)bb.re - out.re, (R)bb.im - out.im);
+                    y[e] = rr;
+                    acc[0] += (double)rr.re * rr.re + (double)rr.im * rr.im;
+                } else {
+                    y[e] = out;
+                    if (EPI == EPI_DOT) {
+                        acc[0] += (double)xu.re * out.re - (double)xu.im * out.im;
+                        acc[1] += (double)xu.re * out.im + (double)xu.im * out.re;
+                    }
+                }
+            }
+        }
+    }
+    if (EPI == EPI_STORE) return;
+    const int slot_stride = NPART * 4;
+    double tot[2];
+    if (!reduce_last<2>(acc, part + (size_t)rhs * slot_stride, cnt + rhs, tot)) return;
+    if (threadIdx.x != 0) return;
+    if (EPI == EPI_RESID) { rn[rhs] = tot[0]; return; }
+    Scal &S = scal[rhs];
+    if (skip) { S.done = 1; return; }
+    S.sig_re = tot[0];
+    S.sig_im = tot[1];
+    if (tot[0] == 0.0 && tot[1] == 0.0) { S.breakdown = 1; S.conv = 1; S.alpha_re = S.alpha_im = 0; return; }
+    c64 a = cdiv(mk<double>(S.rho_re, S.rho_im), mk<double>(tot[0], tot[1]));
+    S.alpha_re = a.re;
+    S.alpha_im = a.im;
+}
+
+// ------------------------------------------------------- vector kernels
+
+// r -= α t; reduce ||r||² and rᵀr; SQMR scalar update in the last block
+template <typename R>
+__global__ __launch_bounds__(BS) void k_upd_r(cx<R> *__restrict__ rall, const cx<R> *__restrict__ tall, int64_t n,
+                                              int64_t vs, double *part, unsigned *cnt, Scal *scal) {
+    const int rhs = blockIdx.y;
+    Scal &S = scal[rhs];
+    if (S.done || S.conv) return;
+    cx<R> *r = rall + rhs * vs;
+    const cx<R> *t = tall + rhs * vs;
+    cx<R> al = mk<R>((R)S.alpha_re, (R)S.alpha_im);
+    double acc[3] = {0, 0, 0};
+    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+        cx<R> v = r[e] - al * t[e];
+        r[e] = v;
+        acc[0] += (double)v.re * v.re + (double)v.im * v.im;
+        acc[1] += (double)v.re * v.re - (double)v.im * v.im;
+        acc[2] += 2.0 * (double)v.re * v.im;
+    }
+    double tot[3];
+    if (!reduce_last<3>(acc, part + (size_t)rhs * NPART * 4, cnt + rhs, tot)) return;
+    if (threadIdx.x != 0) return;
+    double rn = sqrt(tot[0]);
+    double th1 = rn / S.tau, c = 1.0 / sqrt(1.0 + th1 * th1);
+    S.c2t2 = c * c * S.theta * S.theta;
+    S.c2a_re = c * c * S.alpha_re;
+    S.c2a_im = c * c * S.alpha_im;
+    S.tau = S.tau * th1 * c;
+    c64 rho1 = mk<double>(tot[1], tot[2]);
+    if (S.rho_re == 0.0 && S.rho_im == 0.0) { S.beta_re = S.beta_im = 0; S.breakdown = 1; S.conv = 1; }
+    else {
+        c64 b = cdiv(rho1, mk<double>(S.rho_re, S.rho_im));
+        S.beta_re = b.re;
+        S.beta_im = b.im;
+    }
+    S.rho_re = rho1.re;
+    S.rho_im = rho1.im;
+    S.theta = th1;
+    S.rnorm2 = tot[0];
+    S.it += 1;
+    if (S.tau <= S.target || S.it >= S.itmax || rn == 0.0) S.conv = 1;
+}
+
+// d = c²θ² d + c²α q;  z += d;  q = r + β q
+template <typename R>
+__global__ __launch_bounds__(BS) void k_upd_dzq(cx<R> *__restrict__ dall, cx<R> *__restrict__ zall,
+                                                cx<R> *__restrict__ qall, const cx<R> *__restrict__ rall, int64_t n,
+                                                int64_t vs, const Scal *scal) {
+    const int rhs = blockIdx.y;
+    const Scal &S = scal[rhs];
+    if (S.done) return;
+    cx<R> *d = dall + rhs * vs, *z = zall + rhs * vs, *q = qall + rhs * vs;
+    const cx<R> *r = rall + rhs * vs;
+    R c2t2 = (R)S.c2t2;
+    cx<R> c2a = mk<R>((R)S.c2a_re, (R)S.c2a_im), be = mk<R>((R)S.beta_re, (R)S.beta_im);
+    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+        cx<R> qe = q[e];
+        cx<R> de = c2t2 * d[e] + c2a * qe;
+        d[e] = de;
+        z[e] = z[e] + de;
+        q[e] = r[e] + be * qe;
+    }
+}
+
+// cycle start: c = (Q T) r64 in precision R; r = q = c; d = z = 0
+template <typename R>
+__global__ __launch_bounds__(BS) void k_init(DGrid<R> g, OpData<R> od, const c64 *__restrict__ r64all,
+                                             cx<R> *rall, cx<R> *qall, cx<R> *dall, cx<R> *zall, int64_t n,
+                                             int64_t vs, double *part, unsigned *cnt, Scal *scal, double rel_target,
+                                             const int *active, long long itmax) {
+    const int rhs = blockIdx.y;
+    const c64 *r64 = r64all + rhs * vs;
+    cx<R> *r = rall + rhs * vs, *q = qall + rhs * vs, *d = dall + rhs * vs, *z = zall + rhs * vs;
+    bool act = active[rhs] != 0;
+    double acc[3] = {0, 0, 0};
+    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+        int c;
+        int64_t p;
+        dof2cp(g, e, c, p);
+        int i, j, k;
+        cell3(g, p, i, j, k);
+        cx<R> v = czero<R>();
+        if (act) {
+            c64 l = cvt<double>(od.q[3 * p + c] * tfac(g, c, i, j, k, false));
+            c64 s = l * r64[e];
+            v = mk<R>((R)s.re, (R)s.im);
+        }
+        r[e] = v;
+        q[e] = v;
+        d[e] = czero<R>();
+        z[e] = czero<R>();
+        acc[0] += (double)v.re * v.re + (double)v.im * v.im;
+        acc[1] += (double)v.re * v.re - (double)v.im * v.im;
+        acc[2] += 2.0 * (double)v.re * v.im;
+    }
+    double tot[3];
+    if (!reduce_last<3>(acc, part + (size_t)rhs * NPART * 4, cnt + rhs, tot)) return;
+    if (threadIdx.x != 0) return;
+    Scal &S = scal[rhs];
+    memset(&S, 0, sizeof(Scal));
+    S.tau = sqrt(tot[0]);
+    S.cn = S.tau;
+    S.rho_re = tot[1];
+    S.rho_im = tot[2];
+    S.target = rel_target * S.cn;
+    S.itmax = itmax;
+    if (!act || S.cn == 0.0) { S.conv = 1; S.done = 1; }
+}
+
+// e64 += (Q T⁻¹) z
+template <typename R>
+__global__ __launch_bounds__(BS) void k_upd_e(DGrid<R> g, OpData<R> od, c64 *__restrict__ eall,
+                                              const cx<R> *__restrict__ zall, int64_t n, int64_t vs) {
+    const int rhs = blockIdx.y;
+    c64 *e64 = eall + rhs * vs;
+    const cx<R> *z = zall + rhs * vs;
+    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+        cx<R> ze = z[e];
+        if (ze.re == 0 && ze.im == 0) continue;
+        int c;
+        int64_t p;
+        dof2cp(g, e, c, p);
+        int i, j, k;
+        cell3(g, p, i, j, k);
+        c64 rf = cvt<double>(od.q[3 * p + c] * tfac(g, c, i, j, k, true));
+        e64[e] = e64[e] + rf * cvt<double>(ze);
+    }
+}
+
+// ||b||² per RHS
+__global__ __launch_bounds__(BS) void k_norm2(const c64 *__restrict__ ball, int64_t n, int64_t vs, double *part,
+                                              unsigned *cnt, double *out) {
+    const int rhs = blockIdx.y;
+    const c64 *b = ball + rhs * vs;
+    double acc[1] = {0};
+    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS)
+        acc[0] += b[e].re * b[e].re + b[e].im * b[e].im;
+    double tot[1];
+    if (!reduce_last<1>(acc, part + (size_t)rhs * NPART * 4, cnt + rhs, tot)) return;
+    if (threadIdx.x == 0) out[rhs] = tot[0];
+}
+
+// y = s ∘ x with s = T² (S_E, inv = false) or T⁻² (inv = true), in place
+__global__ __launch_bounds__(BS) void k_scale_S(DGrid<double> g, c64 *x, int64_t n, int inv) {
+    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+        int c;
+        int64_t p;
+        dof2cp(g, e, c, p);
+        int i, j, k;
+        cell3(g, p, i, j, k);
+        c64 t = tfac(g, c, i, j, k, inv != 0);
+        x[e] = (t * t) * x[e];
+    }
+}
+
+// grad = 2 Re(ω² λ e)
+__global__ __launch_bounds__(BS) void k_grad(const c64 *e, const c64 *lam, c64 om2, double *grad, int64_t n) {
+    for (int64_t k = blockIdx.x * (int64_t)BS + threadIdx.x; k < n; k += (int64_t)gridDim.x * BS) {
+        c64 v = om2 * (lam[k] * e[k]);
+        grad[k] = 2.0 * v.re;
+    }
+}
+
+// ----------------------------------------------------- operator set-up
+
+// pack (Nx,Ny,Nz,3,3) arrays into per-cell triples
+__global__ void k_pack(const c64 *eps, const c64 *mu, int64_t M, c64 *epsd, c64 *epso, c64 *muinv, int *asym) {
+    int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= M) return;
+    for (int c = 0; c < 3; ++c) {
+        epsd[3 * p + c] = eps[p + M * (c + 3 * c)];
+        if (muinv) muinv[3 * p + c] = cinv(mu[p + M * (c + 3 * c)]);
+    }
+    if (epso) {
+        const int vv[3] = {0, 0, 1}, ww[3] = {1, 2, 2};
+        for (int x = 0; x < 3; ++x) {
+            c64 a = eps[p + M * (vv[x] + 3 * ww[x])], b = eps[p + M * (ww[x] + 3 * vv[x])];
+            epso[3 * p + x] = a;
+            if (a.re != b.re || a.im != b.im) atomicOr(asym, 1);
+        }
+    }
+}
+
+// Jacobi scaling q = (c − ω² ε_cc)^{-1/2}
+template <bool MU>
+__global__ void k_jacobi(DGrid<double> g, OpData<double> od, c64 *q, int use) {
+    int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= g.M) return;
+    int i, j, k;
+    cell3(g, p, i, j, k);
+    for (int c = 0; c < 3; ++c) {
+        if (!use) { q[3 * p + c] = mk<double>(1.0, 0.0); continue; }
+        c64 dg = ccdiag<double, MU>(g, od, c, i, j, k) - od.om2 * od.epsd[3 * p + c];
+        q[3 * p + c] = cinv(csqrt_(dg));
+    }
+}
+
+__global__ void k_down(const c64 *a, c32 *b, int64_t n) {
+    int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e < n) b[e] = mk<float>((float)a[e].re, (float)a[e].im);
+}
+__global__ void k_up(const c32 *a, c64 *b, int64_t n) {
+    int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e < n) b[e] = mk<double>((double)a[e].re, (double)a[e].im);
+}
+
+static inline unsigned nb(int64_t n, int bs = BS) { return (unsigned)((n + bs - 1) / bs); }
+static inline unsigned ngrid(int64_t n) {
+    int64_t b = (n + BS - 1) / BS;
+    return (unsigned)(b < NPART ? (b < 1 ? 1 : b) : NPART);
+}
+
+void Workspace::release() {
+    if (graph) hipGraphExecDestroy(graph);
+    graph = nullptr;
+    if (buf) hipFree(buf);
+    buf = nullptr;
+    if (scal_h) hipHostFree(scal_h);
+    scal_h = nullptr;
+    if (rn_h) hipHostFree(rn_h);
+    rn_h = nullptr;
+    nrhs = 0;
+}
+
+static int ws_ensure(mfd_op *op, int64_t nrhs, int prec) {
+    Workspace &w = op->ws;
+    int64_t n = 3 * op->G.h.M;
+    if (w.buf && w.nrhs == nrhs && w.prec == prec) return MFD_OK;
+    w.release();
+    size_t es = prec == MFD_MIXED ? sizeof(c32) : sizeof(c64);
+    size_t vb = es * n * nrhs;
+    size_t align = 256;
+    auto up = [&](size_t x) { return (x + align - 1) / align * align; };
+    size_t bytes = 5 * up(vb) + up(sizeof(c64) * n * nrhs) + up(sizeof(double) * 3 * nrhs * NPART * 4) +
+                   up(sizeof(unsigned) * 3 * nrhs) + up(sizeof(Scal) * nrhs) + 2 * up(sizeof(double) * nrhs) +
+                   up(sizeof(int) * nrhs);
+    MFD_HIP(hipMalloc(&w.buf, bytes));
+    char *p = (char *)w.buf;
+    w.r = p; p += up(vb);
+    w.q = p; p += up(vb);
+    w.t = p; p += up(vb);
+    w.d = p; p += up(vb);
+    w.z = p; p += up(vb);
+    w.r64 = p; p += up(sizeof(c64) * n * nrhs);
+    w.part = (double *)p; p += up(sizeof(double) * 3 * nrhs * NPART * 4);
+    w.cnt = (unsigned *)p; p += up(sizeof(unsigned) * 3 * nrhs);
+    w.scal = (Scal *)p; p += up(sizeof(Scal) * nrhs);
+    w.bn = (double *)p; p += up(sizeof(double) * nrhs);
+    w.rn = (double *)p; p += up(sizeof(double) * nrhs);
+    w.active = (int *)p; p += up(sizeof(int) * nrhs);
+    MFD_HIP(hipMemsetAsync(w.cnt, 0, sizeof(unsigned) * 3 * nrhs, op->ctx->stream));
+    MFD_HIP(hipHostMalloc((void **)&w.scal_h, sizeof(Scal) * nrhs, hipHostMallocDefault));
+    MFD_HIP(hipHostMalloc((void **)&w.rn_h, sizeof(double) * nrhs, hipHostMallocDefault));
+    w.nrhs = nrhs;
+    w.n = n;
+    w.prec = prec;
+    w.bytes = bytes;
+    w.graph_iters = 0;
+    return MFD_OK;
+}
+
+template <typename R> inline const OpData<R> &opd(const mfd_op *op);
+template <> inline const OpData<double> &opd<double>(const mfd_op *op) { return op->d64; }
+template <> inline const OpData<float> &opd<float>(const mfd_op *op) { return op->d32; }
+
+// launch one operator apply
+template <typename R, int MODE, int EPI>
+static void launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int64_t nrhs, double *part, unsigned *cnt,
+                         Scal *scal, double *rn) {
+    const DGrid<R> &g = op->G.view<R>();
+    const OpData<R> &od = opd<R>(op);
+    dim3 grid(ngrid(g.M), (unsigned)nrhs);
+    int64_t vs = 3 * g.M;
+    hipStream_t s = op->ctx->stream;
+    if (op->full) {
+        if (op->has_mu) k_apply<R, MODE, true, true, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, cnt, scal, rn);
+        else k_apply<R, MODE, true, false, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, cnt, scal, rn);
+    } else {
+        if (op->has_mu) k_apply<R, MODE, false, true, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, cnt, scal, rn);
+        else k_apply<R, MODE, false, false, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, cnt, scal, rn);
+    }
+}
+
+template <typename R>
+static void launch_iter(mfd_op *op, int64_t nrhs) {
+    Workspace &w = op->ws;
+    int64_t n = w.n;
+    hipStream_t s = op->ctx->stream;
+    dim3 gv(ngrid(n), (unsigned)nrhs);
+    size_t pslot = (size_t)nrhs * NPART * 4;
+    launch_apply<R, MODE_P, EPI_DOT>(op, (const cx<R> *)w.q, (cx<R> *)w.t, nullptr, nrhs, w.part, w.cnt, w.scal, nullptr);
+    k_upd_r<R><<<gv, BS, 0, s>>>((cx<R> *)w.r, (const cx<R> *)w.t, n, n, w.part + pslot, w.cnt + nrhs, w.scal);
+    k_upd_dzq<R><<<gv, BS, 0, s>>>((cx<R> *)w.d, (cx<R> *)w.z, (cx<R> *)w.q, (const cx<R> *)w.r, n, n, w.scal);
+}
+
+}  // namespace mfd
+
+using namespace mfd;
+
+// ------------------------------------------------------------- C-ABI glue
+
+int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr,
+                   const mfd_c64 *mu_arr, mfd_op **out) {
+    if (ft != MFD_EE) { set_error("matrix-free operator: only ft = EE is implemented"); return MFD_ERR_UNSUPPORTED; }
+    for (int a = 0; a < 3; ++a)
+        if (g->boundft[a] != MFD_EE) { set_error("matrix-free operator: boundft must be EE on every axis"); return MFD_ERR_UNSUPPORTED; }
+    mfd_op *op = new mfd_op();
+    op->ctx = ctx;
+    int rc = op->G.create(g, ctx->stream);
+    if (rc) { delete op; return rc; }
+    hipStream_t s = ctx->stream;
+    int64_t M = op->G.h.M;
+    op->omega = mk<double>(omega.re, omega.im);
+    c64 om2 = op->omega * op->omega;
+    DevArray E, U;
+    int full = 0, mufull = 0;
+    auto bail = [&](int code) { delete op; return code; };
+    if ((rc = upload_param(ctx, op->G, eps_arr, E, &full))) return bail(rc);
+    if ((rc = upload_param(ctx, op->G, mu_arr, U, &mufull))) return bail(rc);
+    if (mufull) { set_error("create_A: Pμ must be diagonal (Pμ \\ Cₑ), model.jl:236"); return bail(MFD_ERR_UNSUPPORTED); }
+    // is μ identically 1?
+    {
+        std::vector<int> dummy;
+        bool ones = true;
+        for (int c = 0; c < 3 && ones; ++c)
+            for (int64_t p = 0; p < M; ++p) {
+                const mfd_c64 &m = mu_arr[p + M * (c + 3 * c)];
+                if (m.re != 1.0 || m.im != 0.0) { ones = false; break; }
+            }
+        op->has_mu = !ones;
+    }
+    op->full = full;
+    // device arrays: epsd, epso, muinv, q (FP64) and their FP32 copies, + asym flag
+    size_t n3 = 3 * (size_t)M;
+    size_t b64 = sizeof(c64) * n3, b32 = sizeof(c32) * n3;
+    size_t bytes = 4 * (b64 + b32) + 256;
+    if (hipMalloc(&op->buf, bytes) != hipSuccess) { set_error("hipMalloc failed for operator arrays"); return bail(MFD_ERR_ALLOC); }
+    char *p = (char *)op->buf;
+    c64 *epsd = (c64 *)p; p += b64;
+    c64 *epso = (c64 *)p; p += b64;
+    c64 *muinv = (c64 *)p; p += b64;
+    c64 *q = (c64 *)p; p += b64;
+    c32 *epsd32 = (c32 *)p; p += b32;
+    c32 *epso32 = (c32 *)p; p += b32;
+    c32 *muinv32 = (c32 *)p; p += b32;
+    c32 *q32 = (c32 *)p; p += b32;
+    int *asym = (int *)p;
+    if (hipMemsetAsync(asym, 0, sizeof(int), s) != hipSuccess) return bail(MFD_ERR_HIP);
+    k_pack<<<nb(M), BS, 0, s>>>((const c64 *)E.p, (const c64 *)U.p, M, epsd, full ? epso : nullptr,
+                                op->has_mu ? muinv : nullptr, asym);
+    int asym_h = 0;
+    if (hipMemcpyAsync(&asym_h, asym, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) { set_error("operator pack failed"); return bail(MFD_ERR_HIP); }
+    if (asym_h) { set_error("matrix-free operator: off-diagonal ε must be symmetric"); return bail(MFD_ERR_UNSUPPORTED); }
+    op->d64.epsd = epsd;
+    op->d64.epso = full ? epso : nullptr;
+    op->d64.muinv = op->has_mu ? muinv : nullptr;
+    op->d64.q = q;
+    op->d64.om2 = om2;
+    if (op->has_mu) k_jacobi<true><<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q, 1);
+    else k_jacobi<false><<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q, 1);
+    k_down<<<nb(n3), BS, 0, s>>>(epsd, epsd32, n3);
+    if (full) k_down<<<nb(n3), BS, 0, s>>>(epso, epso32, n3);
+    if (op->has_mu) k_down<<<nb(n3), BS, 0, s>>>(muinv, muinv32, n3);
+    k_down<<<nb(n3), BS, 0, s>>>(q, q32, n3);
+    op->d32.epsd = epsd32;
+    op->d32.epso = full ? epso32 : nullptr;
+    op->d32.muinv = op->has_mu ? muinv32 : nullptr;
+    op->d32.q = q32;
+    op->d32.om2 = mk<float>((float)om2.re, (float)om2.im);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) { set_error("operator set-up failed"); return bail(MFD_ERR_HIP); }
+    hipEventCreate(&op->ev0);
+    hipEventCreate(&op->ev1);
+    *out = op;
+    return MFD_OK;
+}
+
+int op_destroy_impl(mfd_op *op) {
+    if (!op) return MFD_OK;
+    hipStreamSynchronize(op->ctx->stream);
+    op->ws.release();
+    if (op->buf) hipFree(op->buf);
+    if (op->ev0) hipEventDestroy(op->ev0);
+    if (op->ev1) hipEventDestroy(op->ev1);
+    delete op;
+    return MFD_OK;
+}
+
+int op_apply_dev_impl(mfd_op *op, const void *x, void *y, int64_t nrhs, int prec) {
+    if (nrhs < 1 || nrhs > 65535) { set_error("nrhs out of range"); return MFD_ERR_ARG; }
+    if (prec == MFD_FP64) launch_apply<double, MODE_A, EPI_STORE>(op, (const c64 *)x, (c64 *)y, nullptr, nrhs, nullptr, nullptr, nullptr, nullptr);
+    else launch_apply<float, MODE_A, EPI_STORE>(op, (const c32 *)x, (c32 *)y, nullptr, nrhs, nullptr, nullptr, nullptr, nullptr);
+    MFD_HIP(hipGetLastError());
+    MFD_HIP(hipStreamSynchronize(op->ctx->stream));
+    return MFD_OK;
+}
+
+int op_apply_impl(mfd_op *op, const mfd_c64 *x, mfd_c64 *y, int64_t nrhs) {
+    int64_t n = 3 * op->G.h.M;
+    size_t b = sizeof(c64) * n * nrhs;
+    void *dx = nullptr, *dy = nullptr;
+    MFD_HIP(hipMalloc(&dx, b));
+    if (hipMalloc(&dy, b) != hipSuccess) { hipFree(dx); set_error("hipMalloc failed"); return MFD_ERR_ALLOC; }
+    hipStream_t s = op->ctx->stream;
+    int rc = MFD_OK;
+    if (hipMemcpyAsync(dx, x, b, hipMemcpyHostToDevice, s) != hipSuccess) rc = MFD_ERR_HIP;
+    if (!rc) rc = op_apply_dev_impl(op, dx, dy, nrhs, MFD_FP64);
+    if (!rc && (hipMemcpyAsync(y, dy, b, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess))
+        rc = MFD_ERR_HIP;
+    hipFree(dx);
+    hipFree(dy);
+    return rc;
+}
+
+int op_bench_apply_impl(mfd_op *op, int64_t nrhs, int prec, int iters, double *ms) {
+    int64_t n = 3 * op->G.h.M;
+    size_t es = prec == MFD_FP64 ? sizeof(c64) : sizeof(c32);
+    void *dx = nullptr, *dy = nullptr;
+    MFD_HIP(hipMalloc(&dx, es * n * nrhs));
+    if (hipMalloc(&dy, es * n * nrhs) != hipSuccess) { hipFree(dx); return MFD_ERR_ALLOC; }
+    hipStream_t s = op->ctx->stream;
+    hipMemsetAsync(dx, 0, es * n * nrhs, s);
+    auto one = [&]() {
+        if (prec == MFD_FP64) launch_apply<double, MODE_A, EPI_STORE>(op, (const c64 *)dx, (c64 *)dy, nullptr, nrhs, nullptr, nullptr, nullptr, nullptr);
+        else launch_apply<float, MODE_P, EPI_STORE>(op, (const c32 *)dx, (c32 *)dy, nullptr, nrhs, nullptr, nullptr, nullptr, nullptr);
+    };
+    for (int w = 0; w < 3; ++w) one();
+    hipEventRecord(op->ev0, s);
+    for (int it = 0; it < iters; ++it) one();
+    hipEventRecord(op->ev1, s);
+    hipError_t e = hipEventSynchronize(op->ev1);
+    float t = 0;
+    hipEventElapsedTime(&t, op->ev0, op->ev1);
+    *ms = t / iters;
+    hipFree(dx);
+    hipFree(dy);
+    if (e != hipSuccess) { set_error(hipGetErrorString(e)); return MFD_ERR_HIP; }
+    return MFD_OK;
+}
+
+void solve_opts_default(mfd_solve_opts *o) {
+    o->tol = 1e-8;
+    o->maxit = 200000;
+    o->precision = MFD_FP64;
+    o->jacobi = 1;
+    o->inner_tol = 1e-4;
+    o->check_every = 32;
+    o->use_graph = 1;
+}
+
+template <typename R>
+static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double rel_target, const int *active_d,
+                     long long budget, long long *iters_out) {
+    Workspace &w = op->ws;
+    hipStream_t s = op->ctx->stream;
+    int64_t n = w.n;
+    dim3 gv(ngrid(n), (unsigned)nrhs);
+    size_t pslot = (size_t)nrhs * NPART * 4;
+    const DGrid<R> &g = op->G.view<R>();
+    const OpData<R> &od = opd<R>(op);
+    k_init<R><<<gv, BS, 0, s>>>(g, od, (const c64 *)w.r64, (cx<R> *)w.r, (cx<R> *)w.q, (cx<R> *)w.d, (cx<R> *)w.z, n, n,
+                                w.part + 2 * pslot, w.cnt + 2 * nrhs, w.scal, rel_target, active_d, budget);
+    MFD_HIP(hipGetLastError());
+    int chunk = o->check_every > 0 ? o->check_every : 32;
+    bool use_graph = o->use_graph != 0;
+    if (use_graph && w.graph_iters != chunk) {
+        if (w.graph) { hipGraphExecDestroy(w.graph); w.graph = nullptr; }
+        hipGraph_t gr;
+        MFD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int it = 0; it < chunk; ++it) launch_iter<R>(op, nrhs);
+        MFD_HIP(hipStreamEndCapture(s, &gr));
+        MFD_HIP(hipGraphInstantiate(&w.graph, gr, nullptr, nullptr, 0));
+        hipGraphDestroy(gr);
+        w.graph_iters = chunk;
+    }
+    long long done_it = 0;
+    while (true) {
+        if (use_graph) MFD_HIP(hipGraphLaunch(w.graph, s));
+        else
+            for (int it = 0; it < chunk; ++it) launch_iter<R>(op, nrhs);
+        MFD_HIP(hipGetLastError());
+        MFD_HIP(hipMemcpyAsync(w.scal_h, w.scal, sizeof(Scal) * nrhs, hipMemcpyDeviceToHost, s));
+        MFD_HIP(hipStreamSynchronize(s));
+        done_it += chunk;
+        bool all = true;
+        long long mx = 0;
+        for (int64_t r = 0; r < nrhs; ++r) {
+            if (!w.scal_h[r].conv) all = false;
+            if (w.scal_h[r].it > mx) mx = w.scal_h[r].it;
+        }
+        *iters_out = mx;
+        if (all || mx >= budget) break;
+    }
+    return MFD_OK;
+}
+
+// e (device, FP64, nrhs) solves A e = b (device) to o->tol.
+int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_solve_opts *oin, mfd_solve_stats *st) {
+    mfd_solve_opts o;
+    if (oin) o = *oin; else solve_opts_default(&o);
+    if (nrhs < 1 || nrhs > 65535) { set_error("nrhs out of range"); return MFD_ERR_ARG; }
+    int prec = o.precision == MFD_MIXED ? MFD_MIXED : MFD_FP64;
+    MFD_TRY(ws_ensure(op, nrhs, prec));
+    Workspace &w = op->ws;
+    hipStream_t s = op->ctx->stream;
+    int64_t n = w.n;
+    size_t pslot = (size_t)nrhs * NPART * 4;
+    dim3 gv(ngrid(n), (unsigned)nrhs);
+    if (!o.jacobi) {
+        // q = 1 (T-scaling only)
+        k_jacobi<false><<<nb(op->G.h.M), BS, 0, s>>>(op->G.d64, op->d64, (c64 *)op->d64.q, 0);
+        k_down<<<nb(3 * op->G.h.M), BS, 0, s>>>(op->d64.q, (c32 *)op->d32.q, 3 * op->G.h.M);
+    }
+    hipEventRecord(op->ev0, s);
+    k_norm2<<<gv, BS, 0, s>>>((const c64 *)b, n, n, w.part + 2 * pslot, w.cnt + 2 * nrhs, w.bn);
+    std::vector<double> bn(nrhs);
+    MFD_HIP(hipMemcpyAsync(bn.data(), w.bn, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+    std::vector<int> active(nrhs, 1);
+    int *active_d = w.active;
+    long long total_it = 0, cycles = 0;
+    double worst = 0;
+    bool conv = false;
+    while (true) {
+        launch_apply<double, MODE_A, EPI_RESID>(op, (const c64 *)e, (c64 *)w.r64, (const c64 *)b, nrhs,
+                                                w.part + 2 * pslot, w.cnt + 2 * nrhs, nullptr, w.rn);
+        MFD_HIP(hipGetLastError());
+        MFD_HIP(hipMemcpyAsync(w.rn_h, w.rn, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+        MFD_HIP(hipStreamSynchronize(s));
+        worst = 0;
+        bool all = true;
+        for (int64_t r = 0; r < nrhs; ++r) {
+            double rel = bn[r] > 0 ? sqrt(w.rn_h[r] / bn[r]) : 0.0;
+            active[r] = rel > o.tol;
+            if (active[r]) all = false;
+            if (rel > worst) worst = rel;
+        }
+        if (all) { conv = true; break; }
+        if (total_it >= o.maxit) break;
+        MFD_HIP(hipMemcpyAsync(active_d, active.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s));
+        double target = prec == MFD_MIXED ? o.inner_tol : std::max(0.3 * o.tol / std::max(worst, 1e-300), 1e-15);
+        if (prec == MFD_FP64) target = std::min(target, 0.3);
+        long long it = 0;
+        int rc;
+        if (prec == MFD_MIXED) rc = run_inner<float>(op, nrhs, &o, target, active_d, o.maxit - total_it, &it);
+        else rc = run_inner<double>(op, nrhs, &o, target, active_d, o.maxit - total_it, &it);
+        if (rc) return rc;
+        total_it += it;
+        ++cycles;
+        if (prec == MFD_MIXED) k_upd_e<float><<<gv, BS, 0, s>>>(op->G.d32, op->d32, (c64 *)e, (const c32 *)w.z, n, n);
+        else k_upd_e<double><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (c64 *)e, (const c64 *)w.z, n, n);
+        MFD_HIP(hipGetLastError());
+        if (cycles > 200) break;
+    }
+    hipEventRecord(op->ev1, s);
+    hipEventSynchronize(op->ev1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, op->ev0, op->ev1);
+    if (!o.jacobi) {
+        if (op->has_mu) k_jacobi<true><<<nb(op->G.h.M), BS, 0, s>>>(op->G.d64, op->d64, (c64 *)op->d64.q, 1);
+        else k_jacobi<false><<<nb(op->G.h.M), BS, 0, s>>>(op->G.d64, op->d64, (c64 *)op->d64.q, 1);
+        k_down<<<nb(3 * op->G.h.M), BS, 0, s>>>(op->d64.q, (c32 *)op->d32.q, 3 * op->G.h.M);
+        hipStreamSynchronize(s);
+    }
+    if (st) {
+        st->iterations = total_it;
+        st->outer_cycles = cycles;
+        st->relres = worst;
+        st->seconds = ms * 1e-3;
+        st->converged = conv;
+    }
+    if (!conv) { set_error("solve did not reach the tolerance within maxit"); return MFD_ERR_NOCONV; }
+    return MFD_OK;
+}
+
+int solve_impl(mfd_op *op, const mfd_c64 *b, mfd_c64 *e, int64_t nrhs, const mfd_solve_opts *o, mfd_solve_stats *st) {
+    int64_t n = 3 * op->G.h.M;
+    size_t bytes = sizeof(c64) * n * nrhs;
+    void *db = nullptr, *de = nullptr;
+    MFD_HIP(hipMalloc(&db, bytes));
+    if (hipMalloc(&de, bytes) != hipSuccess) { hipFree(db); set_error("hipMalloc failed"); return MFD_ERR_ALLOC; }
+    hipStream_t s = op->ctx->stream;
+    int rc = MFD_OK;
+    if (hipMemcpyAsync(db, b, bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(de, e, bytes, hipMemcpyHostToDevice, s) != hipSuccess)
+        rc = MFD_ERR_HIP;
+    if (!rc) rc = solve_dev_impl(op, db, de, nrhs, o, st);
+    if (rc == MFD_OK || rc == MFD_ERR_NOCONV) {
+        if (hipMemcpyAsync(e, de, bytes, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+            rc = MFD_ERR_HIP;
+    }
+    hipFree(db);
+    hipFree(de);
+    return rc;
+}
+
+int solve_adjoint_impl(mfd_op *op, const mfd_c64 *gvec, mfd_c64 *lam, const mfd_solve_opts *o, mfd_solve_stats *st) {
+    int64_t n = 3 * op->G.h.M;
+    size_t bytes = sizeof(c64) * n;
+    void *db = nullptr, *de = nullptr;
+    MFD_HIP(hipMalloc(&db, bytes));
+    if (hipMalloc(&de, bytes) != hipSuccess) { hipFree(db); return MFD_ERR_ALLOC; }
+    hipStream_t s = op->ctx->stream;
+    int rc = MFD_OK;
+    // Aᵀ λ = g  <=>  A (S⁻¹ λ) = S⁻¹ g
+    if (hipMemcpyAsync(db, gvec, bytes, hipMemcpyHostToDevice, s) != hipSuccess || hipMemsetAsync(de, 0, bytes, s) != hipSuccess)
+        rc = MFD_ERR_HIP;
+    if (!rc) {
+        k_scale_S<<<ngrid(n), BS, 0, s>>>(op->G.d64, (c64 *)db, n, 1);
+        rc = solve_dev_impl(op, db, de, 1, o, st);
+    }
+    if (rc == MFD_OK || rc == MFD_ERR_NOCONV) {
+        k_scale_S<<<ngrid(n), BS, 0, s>>>(op->G.d64, (c64 *)de, n, 0);
+        if (hipMemcpyAsync(lam, de, bytes, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+            rc = MFD_ERR_HIP;
+    }
+    hipFree(db);
+    hipFree(de);
+    return rc;
+}
+
+int adjoint_grad_impl(mfd_op *op, const mfd_c64 *e, const mfd_c64 *lam, double *grad) {
+    int64_t n = 3 * op->G.h.M;
+    void *d = nullptr;
+    MFD_HIP(hipMalloc(&d, (2 * sizeof(c64) + sizeof(double)) * n));
+    hipStream_t s = op->ctx->stream;
+    c64 *de = (c64 *)d, *dl = de + n;
+    double *dg = (double *)(dl + n);
+    int rc = MFD_OK;
+    if (hipMemcpyAsync(de, e, sizeof(c64) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(dl, lam, sizeof(c64) * n, hipMemcpyHostToDevice, s) != hipSuccess)
+        rc = MFD_ERR_HIP;
+    if (!rc) {
+        k_grad<<<ngrid(n), BS, 0, s>>>(de, dl, op->omega * op->omega, dg, n);
+        if (hipMemcpyAsync(grad, dg, sizeof(double) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = MFD_ERR_HIP;
+    }
+    hipFree(d);
+    return rc;
+}
+
+// ------------------------------------------------------------- create_b
+
+namespace mfd {
+// b = −Cₘ (Pμ \ jₘ) − iω jₑ  (model.jl:262-265), one thread per cell
+__global__ __launch_bounds__(BS) void k_create_b(DGrid<double> g, const c64 *__restrict__ mu, const c64 *__restrict__ je,
+                                                 const c64 *__restrict__ jm, c64 iw, int omz, c64 *__restrict__ b) {
+    int64_t p = blockIdx.x * (int64_t)BS + threadIdx.x;
+    if (p >= g.M) return;
+    int i, j, k;
+    cell3(g, p, i, j, k);
+    int id[3] = {i, j, k};
+    auto h = [&](int c, int a) -> c64 {   // (jₘ/μ)_c at p − â (a = -1: at p)
+        int r[3] = {i, j, k};
+        c64 ph = mk<double>(1, 0);
+        if (a >= 0) {
+            r[a] -= 1;
+            if (r[a] < 0) {
+                if (!g.bloch[a]) return mk<double>(0, 0);
+                r[a] += g.N[a];
+                ph = g.phinv[a];
+            }
+        }
+        int64_t q = r[0] + (int64_t)g.N[0] * (r[1] + (int64_t)g.N[1] * r[2]);
+        c64 v = cdiv(jm[dofidx(g, c, q)], mu[q + g.M * (c + 3 * c)]);
+        return v * ph;
+    };
+    for (int u = 0; u < 3; ++u) {
+        int v = u == 0 ? 1 : (u == 1 ? 2 : 0), w = u == 0 ? 2 : (u == 1 ? 0 : 1);
+        c64 y = g.sdlinv[PRIM][v][id[v]] * (h(w, -1) - h(w, v)) - g.sdlinv[PRIM][w][id[w]] * (h(v, -1) - h(v, w));
+        int64_t e = dofidx(g, u, p);
+        c64 out = -y;
+        if (!omz) out = out - iw * je[e];
+        b[e] = out;
+    }
+}
+}  // namespace mfd
+
+int create_b_impl(mfd_ctx *ctx, const mfd::DevGrid &G, mfd_c64 omega, const mfd_c64 *mu_arr, const mfd_c64 *je,
+                  const mfd_c64 *jm, mfd_c64 *b) {
+    using namespace mfd;
+    int64_t M = G.h.M, n = 3 * M;
+    hipStream_t s = ctx->stream;
+    DevArray U;
+    int mufull = 0;
+    MFD_TRY(upload_param(ctx, G, mu_arr, U, &mufull));
+    if (mufull) { set_error("create_b: Pμ must be diagonal (Pμ \\ jₘ)"); return MFD_ERR_UNSUPPORTED; }
+    void *d = nullptr;
+    MFD_HIP(hipMalloc(&d, sizeof(c64) * 3 * n));
+    c64 *dje = (c64 *)d, *djm = dje + n, *db = djm + n;
+    int rc = MFD_OK;
+    if (hipMemcpyAsync(dje, je, sizeof(c64) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(djm, jm, sizeof(c64) * n, hipMemcpyHostToDevice, s) != hipSuccess)
+        rc = MFD_ERR_HIP;
+    if (!rc) {
+        c64 iw = mk<double>(-omega.im, omega.re);
+        int omz = omega.re == 0.0 && omega.im == 0.0;
+        k_create_b<<<(unsigned)((M + BS - 1) / BS), BS, 0, s>>>(G.d64, (const c64 *)U.p, dje, djm, iw, omz, db);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(b, db, sizeof(c64) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = MFD_ERR_HIP;
+    }
+    if (rc) set_error("create_b failed");
+    hipFree(d);
+    return rc;
+}

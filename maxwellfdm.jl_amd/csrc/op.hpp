@@ -1,0 +1,62 @@
+// Matrix-free operator and device-resident Krylov solver.
+#pragma once
+#include "grid.hpp"
+
+namespace mfd {
+
+// Per-RHS solver scalars, kept on the device.
+struct Scal {
+    double rho_re, rho_im;     // rᵀr (bilinear)
+    double sig_re, sig_im;     // qᵀ P q
+    double alpha_re, alpha_im;
+    double beta_re, beta_im;
+    double tau, theta, cn, target;
+    double c2t2, c2a_re, c2a_im;
+    double rnorm2;             // scratch: last reduced norm²
+    int conv, done, breakdown, pad;
+    long long it, itmax;
+};
+
+template <typename R>
+struct OpData {
+    const cx<R> *epsd;    // [M][3] diagonal ε at E_c locations
+    const cx<R> *epso;    // [M][3] (xy, xz, yz) at primal corners, or null
+    const cx<R> *muinv;   // [M][3] 1/μ_cc at H_c locations, or null
+    const cx<R> *q;       // [M][3] Jacobi scaling diag(A)^-1/2
+    cx<R> om2;            // ω²
+};
+
+struct Workspace {
+    int64_t nrhs = 0, n = 0;
+    int prec = -1;
+    void *buf = nullptr;
+    size_t bytes = 0;
+    // inner vectors (precision of the inner solve), nrhs each
+    void *r = nullptr, *q = nullptr, *t = nullptr, *d = nullptr, *z = nullptr;
+    void *r64 = nullptr;      // outer FP64 residual
+    double *part = nullptr;   // reduction partials [slot][rhs][NPART][4]
+    unsigned *cnt = nullptr;  // [slot][rhs]
+    Scal *scal = nullptr;     // [rhs]
+    Scal *scal_h = nullptr;   // pinned host mirror
+    double *bn = nullptr;     // [rhs] ||b||²
+    double *rn = nullptr;     // [rhs] ||r||²
+    double *rn_h = nullptr;   // pinned
+    int *active = nullptr;    // [rhs] RHS still above the tolerance
+    hipGraphExec_t graph = nullptr;
+    int graph_iters = 0;
+    void release();
+};
+
+}  // namespace mfd
+
+struct mfd_op {
+    mfd_ctx *ctx;
+    mfd::DevGrid G;
+    mfd::c64 omega;
+    int full = 0, has_mu = 0;
+    void *buf = nullptr;
+    mfd::OpData<double> d64;
+    mfd::OpData<float> d32;
+    mfd::Workspace ws;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};

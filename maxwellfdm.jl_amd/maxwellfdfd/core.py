@@ -1,0 +1,488 @@
+"""Host-side mirror of the MaxwellFDFD assembly and solve API.
+
+Names and argument meaning follow the reference package's exports
+(src/model/model.jl:23-28) with Julia's `!` dropped and Unicode superscripts
+spelled out (create_e⁻ⁱᵏᴸ -> create_eikL).  Every numerical entry point runs
+on the GPU through libmfdfd's C ABI; the Python layer only builds argument
+structs and wraps outputs (CSC results become scipy.sparse.csc_matrix with
+0-based indices; the raw 1-based Julia CSC is available from the *_csc
+functions).
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import List, Tuple
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, context, lib, ptr
+
+EE, HH = 0, 1            # FieldType
+PRIM, DUAL = 0, 1        # GridType
+nX, nY, nZ = 0, 1, 2
+
+
+# --------------------------------------------------------------------- grid
+
+class Grid:
+    """Staggered 3D grid from primal plane locations (MaxwellBase Grid(lprim, isbloch)).
+
+    lprim[w] holds N[w]+1 primal planes including both bounds
+    (test/source.jl:141-142: -10:10 gives 20 cells).  Conventions of the
+    reference's use of Grid (test/source.jl:6-27, pointsrc.jl:66-68):
+    l[PRIM][w] = primal vertices without the ghost at the positive bound;
+    l[DUAL][w] = midpoints; ∆l[PRIM][w] = diff(dual incl. the negative ghost);
+    ∆l[DUAL][w] = diff(primal incl. the positive ghost).
+    """
+
+    def __init__(self, lprim, isbloch=(False, False, False)):
+        self.lprim = tuple(np.ascontiguousarray(np.asarray(a, dtype=np.float64)) for a in lprim)
+        if len(self.lprim) != 3:
+            raise ValueError("Grid needs 3 axes")
+        self.isbloch = tuple(bool(b) for b in isbloch)
+        self.N = tuple(len(a) - 1 for a in self.lprim)
+        if min(self.N) < 1:
+            raise ValueError("each axis needs at least one cell")
+        for a in self.lprim:
+            if np.any(np.diff(a) <= 0):
+                raise ValueError("lprim must be strictly increasing")
+        self.bounds = (np.array([a[0] for a in self.lprim]), np.array([a[-1] for a in self.lprim]))
+        self.L = self.bounds[1] - self.bounds[0]
+        lp = [a[:-1].copy() for a in self.lprim]
+        ld = [0.5 * (a[:-1] + a[1:]) for a in self.lprim]
+        ghost = [ld[w][-1] - self.L[w] if self.isbloch[w] else 2.0 * self.lprim[w][0] - ld[w][0] for w in range(3)]
+        ldg = [np.concatenate(([ghost[w]], ld[w])) for w in range(3)]
+        self.l = (tuple(lp), tuple(ld))
+        self.dl = (tuple(np.diff(ldg[w]) for w in range(3)), tuple(np.diff(self.lprim[w]) for w in range(3)))
+        self.ghosted_l = (tuple(self.lprim), tuple(ldg))
+
+    @property
+    def M(self):
+        return int(np.prod(self.N))
+
+    def __repr__(self):
+        return f"Grid(N={self.N}, isbloch={self.isbloch})"
+
+
+# ------------------------------------------------------------------- shapes
+
+@dataclass
+class Ball:
+    """GeometryPrimitives Ball(c, r)."""
+    c: Tuple[float, float, float]
+    r: float
+
+
+@dataclass
+class Cuboid:
+    """Axis-aligned GeometryPrimitives Cuboid(c, s) (centre, side lengths)."""
+    c: Tuple[float, float, float]
+    s: Tuple[float, float, float]
+
+
+Sphere, Box = Ball, Cuboid
+
+
+def _tensor(v):
+    a = np.asarray(v, dtype=np.complex128)
+    if a.ndim == 0:
+        return np.eye(3, dtype=np.complex128) * a
+    if a.shape == (3,):
+        return np.diag(a)
+    if a.shape == (3, 3):
+        return a.copy()
+    raise ValueError("material parameter must be a scalar, a 3-vector (diagonal) or a 3x3 tensor")
+
+
+@dataclass
+class Material:
+    name: str
+    eps: np.ndarray
+    mu: np.ndarray
+
+
+@dataclass
+class Object:
+    shape: object
+    mat: Material
+
+
+# ------------------------------------------------------------------ sources
+
+class PointSrc:
+    """Point dipole PointSrc(c, p, I∆r) (pointsrc.jl:51-62); p is normalised."""
+
+    def __init__(self, c, p, Idr=1.0):
+        self.c = np.asarray(c, dtype=np.float64)
+        p = np.asarray(p, dtype=np.float64)
+        self.p = p / np.sqrt(np.sum(p * p))
+        self.Idr = complex(Idr)
+
+
+def distweights(c, gt, bounds, l, dl, isbloch):
+    """source.jl:153-154/208-306 on the C ABI: returns ((ind1, ind2), (wt1, wt2)),
+    1-based indices; raises MfdArgumentError where the reference throws."""
+    l = np.ascontiguousarray(l, dtype=np.float64)
+    dl = np.ascontiguousarray(dl, dtype=np.float64)
+    b = (C.c_double * 2)(float(bounds[0]), float(bounds[1]))
+    ind = (C.c_int64 * 2)()
+    wt = (C.c_double * 2)()
+    check(lib().mfd_distweights(float(c), int(gt), b, ptr(l), ptr(dl), len(l), int(bool(isbloch)), ind, wt))
+    return (ind[0], ind[1]), (wt[0], wt[1])
+
+
+def create_field_array(N):
+    return np.zeros(tuple(N) + (3,), dtype=np.complex128, order="F")
+
+
+def create_param_array(N):
+    return np.zeros(tuple(N) + (3, 3), dtype=np.complex128, order="F")
+
+
+# -------------------------------------------------------------------- model
+
+class Model:
+    """MaxwellFDFD.Model for the full 3D equations (model.jl:32-73, full.jl:4-13)."""
+
+    def __init__(self, grid: Grid):
+        self.grid = grid
+        self.ωpml = 0.0
+        self.boundft = [EE, EE, EE]
+        self.Npml = ((0, 0, 0), (0, 0, 0))
+        self.kbloch = np.zeros(3)
+        self.εarr = create_param_array(grid.N)
+        self.μarr = create_param_array(grid.N)
+        self.jₑarr = create_field_array(grid.N)
+        self.jₘarr = create_field_array(grid.N)
+        self.objs: List[Object] = []
+        self.order_cmpfirst = True
+        self.pml_m = 4.0
+        self.pml_lnR = -16.0
+
+    # C-ABI grid description (holds references to the coordinate arrays)
+    def _desc(self):
+        g = L.mfd_grid()
+        for w in range(3):
+            g.N[w] = self.grid.N[w]
+            g.lprim[w] = self.grid.lprim[w].ctypes.data
+            g.isbloch[w] = int(self.grid.isbloch[w])
+            g.boundft[w] = int(self.boundft[w])
+            g.npml[0][w] = int(self.Npml[0][w])
+            g.npml[1][w] = int(self.Npml[1][w])
+            g.kbloch[w] = float(self.kbloch[w])
+        g.omega_pml = L.cplx(self.ωpml)
+        g.pml_m = self.pml_m
+        g.pml_lnR = self.pml_lnR
+        g.order_cmpfirst = int(bool(self.order_cmpfirst))
+        g._keep = self.grid.lprim
+        return g
+
+    def size(self, ft=EE):
+        return (3,) + tuple(self.grid.N) if self.order_cmpfirst else tuple(self.grid.N) + (3,)
+
+    def length(self, ft=EE):
+        return 3 * self.grid.M
+
+
+ModelFull = Model
+
+
+# setters (model.jl:86-89)
+def set_ωpml(mdl, ωpml):
+    mdl.ωpml = ωpml
+
+
+def set_boundft(mdl, boundft):
+    mdl.boundft = [int(b) for b in boundft]
+
+
+def set_Npml(mdl, Npml):
+    mdl.Npml = (tuple(int(v) for v in Npml[0]), tuple(int(v) for v in Npml[1]))
+
+
+def set_kbloch(mdl, kbloch):
+    mdl.kbloch = np.asarray(kbloch, dtype=np.float64)
+
+
+set_omegapml = set_ωpml
+
+
+def create_eikL(mdl):
+    """create_e⁻ⁱᵏᴸ (model.jl:91)."""
+    return np.exp(-1j * mdl.kbloch * mdl.grid.L)
+
+
+def clear_objs(mdl):
+    mdl.εarr[...] = 0
+    mdl.μarr[...] = 0
+    mdl.objs.clear()
+
+
+def clear_srcs(mdl):
+    mdl.jₑarr[...] = 0
+    mdl.jₘarr[...] = 0
+
+
+def add_obj(mdl, matname, *shapes, ε=1.0, μ=1.0):
+    """add_obj!(mdl, name, shapes...; ε, μ) (model.jl:107-120); later objects win."""
+    if len(shapes) == 1 and isinstance(shapes[0], (list, tuple)):
+        shapes = tuple(shapes[0])
+    mat = Material(matname, _tensor(ε), _tensor(μ))
+    for s in shapes:
+        if not isinstance(s, (Ball, Cuboid)):
+            raise TypeError("shapes must be Ball or Cuboid")
+        mdl.objs.append(Object(s, mat))
+
+
+def _add_src(mdl, ft, src):
+    jarr = mdl.jₑarr if ft == EE else mdl.jₘarr
+    c = (C.c_double * 3)(*src.c)
+    p = (C.c_double * 3)(*src.p)
+    g = mdl._desc()
+    check(lib().mfd_add_pointsrc(C.byref(g), ft, c, p, L.cplx(src.Idr), ptr(jarr)))
+
+
+def add_srce(mdl, src):
+    """add_srce! (model.jl:184-191)."""
+    _add_src(mdl, EE, src)
+
+
+def add_srcm(mdl, src):
+    """add_srcm! (model.jl:193-200)."""
+    _add_src(mdl, HH, src)
+
+
+def field_arr2vec(arr, order_cmpfirst=True):
+    if order_cmpfirst:
+        return np.ascontiguousarray(np.moveaxis(arr, 3, 0)).reshape(-1, order="F").copy()
+    return np.asarray(arr).reshape(-1, order="F").copy()
+
+
+def create_srcs(mdl):
+    """create_srcs (model.jl:202-207): fresh DOF vectors (jₑ, jₘ)."""
+    return field_arr2vec(mdl.jₑarr, mdl.order_cmpfirst), field_arr2vec(mdl.jₘarr, mdl.order_cmpfirst)
+
+
+def create_stretched_dls(mdl):
+    """Stretched ∆l tables ((PRIM x,y,z), (DUAL x,y,z)), complex (model.jl:126)."""
+    g = mdl._desc()
+    N = mdl.grid.N
+    out = np.zeros(2 * sum(N), dtype=np.complex128)
+    check(lib().mfd_stretched_dl(context(), C.byref(g), ptr(out)))
+    res, off = [], 0
+    for gt in range(2):
+        row = []
+        for w in range(3):
+            row.append(out[off:off + N[w]].copy())
+            off += N[w]
+        res.append(tuple(row))
+    return tuple(res)
+
+
+def _csc_call(nnz_fn, csc_fn, args, n):
+    nnz = C.c_int64()
+    check(nnz_fn(*args, C.byref(nnz)))
+    colptr = np.zeros(n + 1, dtype=np.int64)
+    rowval = np.zeros(max(nnz.value, 1), dtype=np.int64)
+    nzval = np.zeros(max(nnz.value, 1), dtype=np.complex128)
+    check(csc_fn(*args, ptr(colptr), ptr(rowval), ptr(nzval)))
+    return colptr, rowval[:nnz.value], nzval[:nnz.value]
+
+
+def to_scipy(csc):
+    import scipy.sparse as sp
+    colptr, rowval, nzval = csc
+    n = len(colptr) - 1
+    return sp.csc_matrix((nzval, rowval - 1, colptr - 1), shape=(n, n))
+
+
+def curl_csc(mdl, which):
+    """Raw Julia-form CSC (1-based colptr, rowval, nzval) of Cₑ (0) or Cₘ (1)."""
+    g = mdl._desc()
+    return _csc_call(lib().mfd_curl_nnz, lib().mfd_curl_csc, (context(), C.byref(g), which), 3 * mdl.grid.M)
+
+
+def paramop_csc(mdl, which, parr):
+    g = mdl._desc()
+    parr = L.carr(parr, "F")
+    return _csc_call(lib().mfd_paramop_nnz, lib().mfd_paramop_csc, (context(), C.byref(g), which, ptr(parr)),
+                     3 * mdl.grid.M)
+
+
+def A_csc(mdl, ω, εarr, μarr, ft=EE):
+    g = mdl._desc()
+    e = L.carr(εarr, "F")
+    u = L.carr(μarr, "F")
+    return _csc_call(lib().mfd_A_nnz, lib().mfd_A_csc, (context(), C.byref(g), ft, L.cplx(ω), ptr(e), ptr(u)),
+                     3 * mdl.grid.M)
+
+
+def _shapes_tables(mdl):
+    n = len(mdl.objs)
+    arr = (L.mfd_shape * max(n, 1))()
+    et = np.zeros((max(n, 1), 9), dtype=np.complex128)
+    mt = np.zeros((max(n, 1), 9), dtype=np.complex128)
+    for i, o in enumerate(mdl.objs):
+        s = o.shape
+        if isinstance(s, Ball):
+            arr[i].kind = 0
+            arr[i].r = float(s.r)
+        else:
+            arr[i].kind = 1
+            for a in range(3):
+                arr[i].s[a] = float(s.s[a])
+        for a in range(3):
+            arr[i].c[a] = float(s.c[a])
+        et[i] = o.mat.eps.reshape(-1, order="F")
+        mt[i] = o.mat.mu.reshape(-1, order="F")
+    return n, arr, et, mt
+
+
+def calc_matparams(mdl):
+    """calc_matparams! (full.jl:16-70): assignment + subpixel smoothing into εarr, μarr."""
+    g = mdl._desc()
+    n, arr, et, mt = _shapes_tables(mdl)
+    check(lib().mfd_matparams(context(), C.byref(g), n, C.cast(arr, C.c_void_p), ptr(et), ptr(mt),
+                              ptr(mdl.εarr), ptr(mdl.μarr)))
+
+
+def create_paramops(mdl):
+    """create_paramops (model.jl:141-158): updates εarr/μarr and returns (Pε, Pμ)."""
+    calc_matparams(mdl)
+    return to_scipy(paramop_csc(mdl, 0, mdl.εarr)), to_scipy(paramop_csc(mdl, 1, mdl.μarr))
+
+
+def create_curls(mdl):
+    """create_curls (model.jl:160-175): (Cₑ, Cₘ)."""
+    return to_scipy(curl_csc(mdl, 0)), to_scipy(curl_csc(mdl, 1))
+
+
+def create_A(ft, ω, mdl, εarr=None, μarr=None):
+    """create_A (model.jl:225-246) assembled on the device in closed form.
+
+    The reference takes the assembled (Pε, Pμ, Cₑ, Cₘ); the device path
+    assembles the same A directly from the model's grid and parameter arrays.
+    """
+    εarr = mdl.εarr if εarr is None else εarr
+    μarr = mdl.μarr if μarr is None else μarr
+    return to_scipy(A_csc(mdl, ω, εarr, μarr, ft))
+
+
+def create_b(ft, ω, mdl, jₑ, jₘ, μarr=None):
+    """create_b (model.jl:251-274): b = −Cₘ (Pμ \\ jₘ) − iω jₑ on the device."""
+    μarr = mdl.μarr if μarr is None else μarr
+    g = mdl._desc()
+    je = L.carr(jₑ)
+    jm = L.carr(jₘ)
+    b = np.zeros_like(je)
+    u = L.carr(μarr, "F")
+    check(lib().mfd_create_b(context(), C.byref(g), ft, L.cplx(ω), ptr(u), ptr(je), ptr(jm), ptr(b)))
+    return b
+
+
+def create_linsys(ft, ω, mdl, jₑ=None, jₘ=None):
+    """create_linsys (model.jl:209-220): (A, b)."""
+    if jₑ is None:
+        jₑ, jₘ = create_srcs(mdl)
+    return create_A(ft, ω, mdl), create_b(ft, ω, mdl, jₑ, jₘ)
+
+
+# ------------------------------------------------------- matrix-free solve
+
+class MaxwellOperator:
+    """Device-resident A = Cₘ μ⁻¹ Cₑ − ω² Pε (model.jl:235-237) with its solver.
+
+    solve() is the user's `A \\ b` of the reference workflow (iterative:
+    Jacobi-scaled symmetric QMR; precision "fp64" or "mixed" = FP32 inner
+    iterations with FP64 residual and update).
+    """
+
+    def __init__(self, mdl, ω, ft=EE, εarr=None, μarr=None):
+        self.mdl = mdl
+        self.ω = complex(ω)
+        self.n = 3 * mdl.grid.M
+        e = L.carr(mdl.εarr if εarr is None else εarr, "F")
+        u = L.carr(mdl.μarr if μarr is None else μarr, "F")
+        g = mdl._desc()
+        h = C.c_void_p()
+        check(lib().mfd_op_create(context(), C.byref(g), ft, L.cplx(ω), ptr(e), ptr(u), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mfd_op_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def apply(self, x):
+        x = L.carr(x)
+        nrhs = x.size // self.n
+        y = np.zeros_like(x)
+        check(lib().mfd_op_apply(self._h, ptr(x), ptr(y), nrhs))
+        return y
+
+    def apply_dev(self, x_ptr, y_ptr, nrhs=1, precision="fp64"):
+        check(lib().mfd_op_apply_dev(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), nrhs,
+                                     L.MFD_FP64 if precision == "fp64" else L.MFD_MIXED))
+
+    def bench_apply(self, nrhs=1, precision="fp64", iters=20):
+        ms = C.c_double()
+        check(lib().mfd_op_bench_apply(self._h, nrhs, L.MFD_FP64 if precision == "fp64" else L.MFD_MIXED, iters,
+                                       C.byref(ms)))
+        return ms.value
+
+    @staticmethod
+    def _opts(tol, precision, maxit, inner_tol, jacobi, check_every, use_graph):
+        o = L.mfd_solve_opts()
+        lib().mfd_solve_opts_default(C.byref(o))
+        o.tol = tol
+        o.precision = L.MFD_MIXED if precision == "mixed" else L.MFD_FP64
+        if maxit is not None:
+            o.maxit = maxit
+        if inner_tol is not None:
+            o.inner_tol = inner_tol
+        o.jacobi = int(bool(jacobi))
+        if check_every is not None:
+            o.check_every = check_every
+        o.use_graph = int(bool(use_graph))
+        return o
+
+    def solve(self, b, tol=1e-8, precision="fp64", x0=None, maxit=None, inner_tol=None, jacobi=True,
+              check_every=None, use_graph=True, raise_on_noconv=True):
+        b = L.carr(b)
+        nrhs = b.size // self.n
+        e = np.zeros_like(b) if x0 is None else L.carr(x0).copy()
+        o = self._opts(tol, precision, maxit, inner_tol, jacobi, check_every, use_graph)
+        st = L.mfd_solve_stats()
+        rc = lib().mfd_solve(self._h, ptr(b), ptr(e), nrhs, C.byref(o), C.byref(st))
+        if rc != L.MFD_OK and (raise_on_noconv or rc != L.MFD_ERR_NOCONV):
+            check(rc)
+        return e, st
+
+    def solve_dev(self, b_ptr, e_ptr, nrhs=1, tol=1e-8, precision="fp64", maxit=None, inner_tol=None,
+                  jacobi=True, check_every=None, use_graph=True):
+        o = self._opts(tol, precision, maxit, inner_tol, jacobi, check_every, use_graph)
+        st = L.mfd_solve_stats()
+        check(lib().mfd_solve_dev(self._h, C.c_void_p(b_ptr), C.c_void_p(e_ptr), nrhs, C.byref(o), C.byref(st)))
+        return st
+
+    def solve_adjoint(self, g, tol=1e-8, precision="fp64", maxit=None):
+        g = L.carr(g)
+        lam = np.zeros_like(g)
+        o = self._opts(tol, precision, maxit, None, True, None, True)
+        st = L.mfd_solve_stats()
+        check(lib().mfd_solve_adjoint(self._h, ptr(g), ptr(lam), C.byref(o), C.byref(st)))
+        return lam, st
+
+    def adjoint_grad(self, e, lam):
+        """∂f/∂ε per E DOF: 2 Re(ω² λ ∘ e) for Aᵀλ = (∂f/∂e)ᵀ."""
+        e = L.carr(e)
+        lam = L.carr(lam)
+        grad = np.zeros(e.size, dtype=np.float64)
+        check(lib().mfd_adjoint_grad(self._h, ptr(e), ptr(lam), ptr(grad)))
+        return grad

@@ -1,0 +1,73 @@
+"""Seeded synthetic problems of the published benchmark configurations.
+
+Every geometry draw uses numpy's PCG64 with an explicit seed, so the CPU
+checker and the device path see identical inputs.
+"""
+import numpy as np
+
+from . import core as M
+
+
+def uniform_grid(N):
+    N = (N, N, N) if np.isscalar(N) else tuple(N)
+    return M.Grid([np.arange(n + 1.0) for n in N])
+
+
+def base_model(N, npml, omega):
+    g = uniform_grid(N)
+    m = M.ModelFull(g)
+    M.set_ωpml(m, omega)
+    M.set_Npml(m, ((npml,) * 3, (npml,) * 3))
+    Ns = g.N
+    M.add_obj(m, "vacuum", M.Cuboid(tuple(n / 2 for n in Ns), tuple(float(n) for n in Ns)), ε=1.0, μ=1.0)
+    return m
+
+
+def sphere_dipole(N=32, npml=8, r=6.0, eps=12.0, lam=20.0):
+    """Plain-method small case: centred sphere ε=12 (r=6), z-dipole at the centre, ω=2π/20."""
+    om = 2 * np.pi / lam
+    m = base_model(N, npml, om)
+    c = (N / 2,) * 3
+    M.add_obj(m, "dielectric", M.Ball(c, r), ε=eps)
+    M.add_srce(m, M.PointSrc(c, (0, 0, 1), 1.0))
+    return m, om
+
+
+def random_dipoles(N, npml, n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        pos = rng.uniform(npml + 2, N - npml - 2, size=3)
+        out.append((pos, v))
+    return out
+
+
+def random_boxes(N=128, npml=10, nbox=40, lam=24.0, seed=1, ndip=1):
+    """Mixed-precision case: background ε=1 with nbox axis-aligned boxes, corners
+    and sizes uniform inside the non-PML interior, ε ~ U[1,12]; dipoles of
+    uniform random orientation at seeded interior positions."""
+    om = 2 * np.pi / lam
+    m = base_model(N, npml, om)
+    rng = np.random.default_rng(seed)
+    lo_i, hi_i = float(npml), float(N - npml)
+    for b in range(nbox):
+        s = rng.uniform(4.0, (hi_i - lo_i) / 3, size=3)
+        lo = rng.uniform(lo_i, hi_i - s)
+        eps = rng.uniform(1.0, 12.0)
+        M.add_obj(m, f"box{b}", M.Cuboid(tuple(lo + s / 2), tuple(s)), ε=eps)
+    for pos, v in random_dipoles(N, npml, ndip, seed + 1000):
+        M.add_srce(m, M.PointSrc(pos, v, 1.0))
+    return m, om
+
+
+def dipole_rhs(mdl, omega, dipoles):
+    """One b vector per dipole (nrhs x n)."""
+    bs = []
+    for pos, v in dipoles:
+        M.clear_srcs(mdl)
+        M.add_srce(mdl, M.PointSrc(pos, v, 1.0))
+        je, jm = M.create_srcs(mdl)
+        bs.append(M.create_b(M.EE, omega, mdl, je, jm))
+    return np.array(bs)

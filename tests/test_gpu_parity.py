@@ -1,0 +1,341 @@
+"""Device path (libmfdfd through its C ABI) against the oracle.
+
+CSC index structure (colptr/rowval) must be bit-exact; values agree to 1e-13
+relative to the matrix's largest entry; the matrix-free operator to 1e-12;
+solved fields to 1e-7 relative at FP64 residual 1e-8 (the tolerances the
+north-star contract states for values and fields).
+"""
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as sla
+
+from conftest import shapes_of, spec_of
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def nonuniform(N, seed):
+    rng = np.random.default_rng(seed)
+    return [np.cumsum(np.concatenate(([0.0], rng.uniform(0.6, 1.4, n)))) for n in N]
+
+
+CASES = {
+    "uniform_pml": dict(N=(6, 6, 6), npml=((2, 2, 2), (2, 2, 2)), om=0.7, bloch=(False,) * 3),
+    "nonuniform_bloch": dict(N=(5, 4, 6), npml=((0,) * 3, (0,) * 3), om=0.9, bloch=(True,) * 3,
+                             kb=(0.3, -0.2, 0.5), nu=1),
+    "mixed_pml_bloch": dict(N=(4, 5, 3), npml=((1, 0, 1), (1, 1, 0)), om=0.6 + 0.05j, bloch=(True, False, True),
+                            kb=(0.1, 0.0, 0.4), nu=2),
+    "tiny_bloch_axes": dict(N=(1, 2, 5), npml=((0, 0, 1), (0, 0, 1)), om=0.5, bloch=(True, True, False),
+                            kb=(0.0, 0.3, 0.0)),
+    "cmplast": dict(N=(4, 3, 5), npml=((1, 1, 1), (1, 1, 1)), om=0.8, bloch=(False,) * 3, cmpfirst=False),
+    "boundft_HH": dict(N=(4, 4, 4), npml=((1, 1, 1), (1, 1, 1)), om=0.8, bloch=(False, True, False),
+                       bft=(0, 1, 0)),
+}
+
+
+def make(pkg, case):
+    c = CASES[case]
+    N = c["N"]
+    lp = nonuniform(N, c["nu"]) if "nu" in c else [np.arange(n + 1.0) for n in N]
+    g = pkg.Grid(lp, c["bloch"])
+    m = pkg.ModelFull(g)
+    pkg.set_ωpml(m, c["om"])
+    pkg.set_Npml(m, c["npml"])
+    pkg.set_kbloch(m, c.get("kb", (0, 0, 0)))
+    pkg.set_boundft(m, c.get("bft", (0, 0, 0)))
+    m.order_cmpfirst = c.get("cmpfirst", True)
+    return m
+
+
+def rand_params(m, seed, full=False, mu=False):
+    rng = np.random.default_rng(seed)
+    N = m.grid.N
+    eps = np.zeros(N + (3, 3), dtype=np.complex128, order="F")
+    muu = np.zeros_like(eps)
+    for c in range(3):
+        eps[..., c, c] = 1 + 11 * rng.random(N) + 0.1j * rng.random(N)
+        muu[..., c, c] = (1 + 0.5 * rng.random(N)) if mu else 1.0
+    if full:
+        for (v, w) in ((0, 1), (0, 2), (1, 2)):
+            x = rng.normal(size=N) + 0.1j * rng.normal(size=N)
+            x[rng.random(N) < 0.5] = 0
+            eps[..., v, w] = x
+            eps[..., w, v] = x
+    return eps, muu
+
+
+def assert_csc_equal(got, ref, rtol, what):
+    gc, gr, gv = got
+    rc, rr, rv = ref
+    assert np.array_equal(gc, rc), f"{what}: colptr differs"
+    assert np.array_equal(gr, rr), f"{what}: rowval differs"
+    scale = max(np.abs(rv).max(), 1e-300)
+    err = np.abs(gv - rv).max() if len(rv) else 0.0
+    assert err <= rtol * scale, f"{what}: values differ by {err / scale:.2e} (relative to max)"
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_stretched_dl(case, pkg, O):
+    m = make(pkg, case)
+    say("stretched_dl", case)
+    got = pkg.create_stretched_dls(m)
+    T = O.grid_tables(spec_of(m))
+    for gt in range(2):
+        for w in range(3):
+            np.testing.assert_allclose(got[gt][w], T[("sdl", gt, w)], rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_curl_csc(case, pkg, O):
+    m = make(pkg, case)
+    s = spec_of(m)
+    for which in (0, 1):
+        say("curl_csc", case, which)
+        assert_csc_equal(pkg.curl_csc(m, which), O.curl(s, which), 1e-14, f"curl{which}")
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("full", [False, True])
+def test_paramop_csc(case, full, pkg, O):
+    m = make(pkg, case)
+    s = spec_of(m)
+    eps, mu = rand_params(m, 3, full=full, mu=True)
+    for which, arr in ((0, eps), (1, mu)):
+        say("paramop_csc", case, full, which)
+        assert_csc_equal(pkg.paramop_csc(m, which, arr), O.paramop(s, which, arr), 1e-13, f"param{which}")
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("full", [False, True])
+def test_A_csc(case, full, pkg, O):
+    m = make(pkg, case)
+    s = spec_of(m)
+    eps, mu = rand_params(m, 4, full=full, mu=True)
+    for om in (0.55, 0.0):
+        say("A_csc", case, full, om)
+        assert_csc_equal(pkg.A_csc(m, om, eps, mu), O.create_A(s, om, eps, mu), 1e-13, "A")
+
+
+def test_A_rejects_nondiagonal_mu_and_HH(pkg):
+    m = make(pkg, "uniform_pml")
+    eps, mu = rand_params(m, 5)
+    mu2 = mu.copy()
+    mu2[..., 0, 1] = 0.1
+    with pytest.raises(pkg.MfdUnsupportedError):
+        pkg.A_csc(m, 0.5, eps, mu2)
+    with pytest.raises(pkg.MfdUnsupportedError):
+        pkg.A_csc(m, 0.5, eps, mu, ft=pkg.HH)
+    m2 = make(pkg, "uniform_pml")
+    pkg.set_ωpml(m2, 0.0)
+    with pytest.raises(pkg.MfdArgumentError):
+        pkg.create_stretched_dls(m2)
+
+
+@pytest.mark.parametrize("case", ["uniform_pml", "mixed_pml_bloch", "cmplast"])
+def test_create_b(case, pkg, O):
+    m = make(pkg, case)
+    s = spec_of(m)
+    eps, mu = rand_params(m, 6, mu=True)
+    rng = np.random.default_rng(1)
+    n = 3 * m.grid.M
+    je = rng.normal(size=n) + 1j * rng.normal(size=n)
+    jm = rng.normal(size=n) + 1j * rng.normal(size=n)
+    for om in (0.7, 0.0):
+        say("create_b", case, om)
+        got = pkg.create_b(pkg.EE, om, m, je, jm, μarr=mu)
+        ref = O.create_b(s, om, mu, je, jm)
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def geom_model(pkg, N=(14, 12, 13), nonuni=True, seed=0):
+    lp = nonuniform(N, 30 + seed) if nonuni else [np.arange(n + 1.0) for n in N]
+    g = pkg.Grid(lp)
+    m = pkg.ModelFull(g)
+    pkg.set_ωpml(m, 0.6)
+    pkg.set_Npml(m, ((2, 2, 2), (2, 2, 2)))
+    L = [a[-1] - a[0] for a in lp]
+    c0 = [0.5 * (a[-1] + a[0]) for a in lp]
+    pkg.add_obj(m, "bg", pkg.Cuboid(tuple(c0), tuple(L)), ε=1.0)
+    rng = np.random.default_rng(seed)
+    for k in range(4):
+        pkg.add_obj(m, f"box{k}", pkg.Cuboid(tuple(rng.uniform(3, 9, 3)), tuple(rng.uniform(1.5, 5, 3))),
+                    ε=rng.uniform(2, 12))
+    pkg.add_obj(m, "ball", pkg.Ball(tuple(c0), 3.7), ε=12.0 + 0.5j)
+    pkg.add_obj(m, "aniso", pkg.Ball((c0[0] + 2, c0[1] - 1, c0[2]), 1.9), ε=np.diag([3.0, 4.0, 5.0]), μ=1.0)
+    return m
+
+
+@pytest.mark.parametrize("nonuni", [False, True])
+def test_matparams(nonuni, pkg, O):
+    m = geom_model(pkg, nonuni=nonuni)
+    say("matparams", nonuni)
+    pkg.calc_matparams(m)
+    shapes, et, mt = shapes_of(m)
+    ea, ma = O.matparams(spec_of(m), shapes, et, mt)
+    assert np.abs(m.εarr - ea).max() <= 1e-12 * np.abs(ea).max()
+    assert np.abs(m.μarr - ma).max() <= 1e-12 * np.abs(ma).max()
+    assert np.abs(ea[..., 0, 1]).max() > 0.1
+
+
+OPCASES = ["uniform_pml", "nonuniform_bloch", "mixed_pml_bloch", "tiny_bloch_axes", "cmplast"]
+
+
+@pytest.mark.parametrize("case", OPCASES)
+@pytest.mark.parametrize("full,mu", [(False, False), (True, False), (False, True), (True, True)])
+def test_apply_matches_csc(case, full, mu, pkg, O):
+    m = make(pkg, case)
+    s = spec_of(m)
+    eps, muu = rand_params(m, 7, full=full, mu=mu)
+    om = 0.55
+    say("apply", case, full, mu)
+    A = O.to_scipy(O.create_A(s, om, eps, muu))
+    op = pkg.MaxwellOperator(m, om, εarr=eps, μarr=muu)
+    rng = np.random.default_rng(2)
+    n = 3 * m.grid.M
+    X = rng.normal(size=(3, n)) + 1j * rng.normal(size=(3, n))
+    Y = op.apply(X)
+    for r in range(3):
+        ref = A @ X[r]
+        assert np.linalg.norm(Y[r] - ref) <= 1e-12 * np.linalg.norm(ref)
+    op.close()
+
+
+def c1_like(pkg, N=12, npml=3, r=2.5, lam=8.0):
+    from maxwellfdfd import workloads as W
+    m, om = W.sphere_dipole(N=N, npml=npml, r=r, lam=lam)
+    pkg.calc_matparams(m)
+    je, jm = pkg.create_srcs(m)
+    b = pkg.create_b(pkg.EE, om, m, je, jm)
+    return m, om, b
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_vs_direct(precision, pkg, O):
+    m, om, b = c1_like(pkg)
+    say("solve vs direct", precision)
+    A = O.to_scipy(O.create_A(spec_of(m), om, m.εarr, m.μarr))
+    op = pkg.MaxwellOperator(m, om)
+    e, st = op.solve(b, tol=1e-8, precision=precision)
+    rel = np.linalg.norm(b - A @ e) / np.linalg.norm(b)
+    say("  iterations", st.iterations, "cycles", st.outer_cycles, "relres", st.relres, "oracle residual", rel)
+    assert st.converged and rel <= 1.05e-8
+    xd = sla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(e - xd) / np.linalg.norm(xd) <= 1e-6
+    op.close()
+
+
+def test_solve_c1_against_oracle(pkg, O):
+    """Plain method, small: 32³, 8-cell PML, ε=12 sphere r=6, z-dipole, ω=2π/20;
+    GPU FP64 solve vs the oracle CPU FP64 solve, both to residual 1e-10."""
+    from maxwellfdfd import workloads as W
+    m, om = W.sphere_dipole()
+    pkg.calc_matparams(m)
+    je, jm = pkg.create_srcs(m)
+    b = pkg.create_b(pkg.EE, om, m, je, jm)
+    s = spec_of(m)
+    shapes, et, mt = shapes_of(m)
+    ea, ma = O.matparams(s, shapes, et, mt)
+    assert np.abs(m.εarr - ea).max() <= 1e-12 * 12
+    A = O.create_A(s, om, ea, ma)
+    say("c1: oracle solve")
+    xo, it_o, rr_o = O.solve_sqmr(s, A, b, tol=1e-10, maxit=50000, nthreads=0)
+    op = pkg.MaxwellOperator(m, om)
+    say("c1: gpu solve")
+    e, st = op.solve(b, tol=1e-10)
+    As = O.to_scipy(A)
+    rel = np.linalg.norm(b - As @ e) / np.linalg.norm(b)
+    say("  gpu its", st.iterations, "res", rel, "cpu its", it_o, "res", rr_o)
+    assert rel <= 1.05e-10 and rr_o <= 1e-10
+    assert np.linalg.norm(e - xo) / np.linalg.norm(xo) <= 1e-7
+    e8, st8 = op.solve(b, tol=1e-8, precision="mixed")
+    rel8 = np.linalg.norm(b - As @ e8) / np.linalg.norm(b)
+    say("  mixed its", st8.iterations, "cycles", st8.outer_cycles, "res", rel8,
+        "field err", np.linalg.norm(e8 - xo) / np.linalg.norm(xo))
+    assert rel8 <= 1.05e-8
+    op.close()
+
+
+def test_solve_batched(pkg, O):
+    from maxwellfdfd import workloads as W
+    m, om = W.random_boxes(N=16, npml=3, nbox=6, lam=8.0, seed=3, ndip=0)
+    pkg.calc_matparams(m)
+    dips = W.random_dipoles(16, 3, 3, 11)
+    B = W.dipole_rhs(m, om, dips)
+    A = O.to_scipy(O.create_A(spec_of(m), om, m.εarr, m.μarr))
+    op = pkg.MaxwellOperator(m, om)
+    for precision in ("fp64", "mixed"):
+        say("batched", precision)
+        E, st = op.solve(B, tol=1e-8, precision=precision)
+        for r in range(3):
+            rel = np.linalg.norm(B[r] - A @ E[r]) / np.linalg.norm(B[r])
+            assert rel <= 1.05e-8
+            e1, _ = op.solve(B[r], tol=1e-8, precision=precision)
+            assert np.linalg.norm(E[r] - e1) / np.linalg.norm(e1) <= 1e-6
+    op.close()
+
+
+def test_solve_without_graph_and_jacobi(pkg, O):
+    m, om, b = c1_like(pkg, N=10, npml=2, r=2.0, lam=7.0)
+    A = O.to_scipy(O.create_A(spec_of(m), om, m.εarr, m.μarr))
+    op = pkg.MaxwellOperator(m, om)
+    for kw in (dict(use_graph=False), dict(jacobi=False), dict(check_every=5)):
+        say("solve options", kw)
+        e, st = op.solve(b, tol=1e-8, **kw)
+        assert np.linalg.norm(b - A @ e) / np.linalg.norm(b) <= 1.05e-8
+    op.close()
+
+
+def test_solve_iteration_cap_reports(pkg):
+    m, om, b = c1_like(pkg, N=10, npml=2, r=2.0, lam=7.0)
+    op = pkg.MaxwellOperator(m, om)
+    say("iteration cap")
+    with pytest.raises(pkg.MfdNoConvergence):
+        op.solve(b, tol=1e-8, maxit=5, check_every=5)
+    e, st = op.solve(b, tol=1e-8, maxit=5, check_every=5, raise_on_noconv=False)
+    assert not st.converged and st.iterations <= 10
+    op.close()
+
+
+def test_adjoint(pkg, O):
+    """Aᵀλ = g against a direct solve of the transpose; ∂f/∂ε against central
+    finite differences (h = 1e-4) for f = Σ|e_z|² over a monitor."""
+    m, om, b = c1_like(pkg, N=10, npml=2, r=2.0, lam=7.0)
+    A = O.to_scipy(O.create_A(spec_of(m), om, m.εarr, m.μarr))
+    op = pkg.MaxwellOperator(m, om)
+    e, _ = op.solve(b, tol=1e-12)
+    N = m.grid.N
+    mon = [2 + 3 * (i + N[0] * (j + N[1] * 7)) for i in range(4, 6) for j in range(4, 6)]
+    g = np.zeros_like(e)
+    g[mon] = np.conj(e[mon])
+    say("adjoint solve")
+    lam, st = op.solve_adjoint(g, tol=1e-12)
+    ref = sla.spsolve(A.T.tocsc(), g)
+    assert np.linalg.norm(lam - ref) / np.linalg.norm(ref) <= 1e-9
+    grad = op.adjoint_grad(e, lam)
+    op.close()
+
+    def f_of(eps_arr):
+        o = pkg.MaxwellOperator(m, om, εarr=eps_arr)
+        x, _ = o.solve(b, tol=1e-13)
+        o.close()
+        return np.sum(np.abs(x[mon]) ** 2)
+
+    rng = np.random.default_rng(0)
+    for t in range(4):
+        c = int(rng.integers(3))
+        i, j, k = (int(v) for v in rng.integers(3, 7, 3))
+        h = 1e-4
+        ep, em = m.εarr.copy(), m.εarr.copy()
+        ep[i, j, k, c, c] += h
+        em[i, j, k, c, c] -= h
+        fd = (f_of(ep) - f_of(em)) / (2 * h)
+        dof = c + 3 * (i + N[0] * (j + N[1] * k))
+        say("  fd", fd, "adjoint", grad[dof])
+        assert grad[dof] == pytest.approx(fd, rel=1e-5, abs=1e-9 * np.abs(grad).max())
