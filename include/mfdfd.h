@@ -71,8 +71,8 @@ typedef struct {
     double tol;               /* relative residual ||b-Ae||/||b|| in FP64 (1e-8) */
     int64_t maxit;            /* total Krylov iterations allowed per RHS */
     int32_t precision;        /* MFD_FP64 or MFD_MIXED (FP32 inner, FP64 refinement) */
-    int32_t jacobi;           /* 1 = Jacobi diagonal scaling (default), 0 = none */
-    double inner_tol;         /* mixed: inner relative reduction per cycle (1e-4) */
+    int32_t pad;
+    double inner_tol;         /* mixed: tightest inner reduction per FP32 cycle (1e-5) */
     int32_t check_every;      /* iterations between host convergence checks (32) */
     int32_t use_graph;        /* capture a chunk of iterations in a hipGraph */
 } mfd_solve_opts;

@@ -13,8 +13,9 @@ struct Scal {
     double tau, theta, cn, target;
     double c2t2, c2a_re, c2a_im;
     double rnorm2;             // scratch: last reduced norm²
-    int conv, done, breakdown, pad;
-    long long it, itmax;
+    int conv, done, breakdown, stag;
+    long long it, itmax, it_ref;
+    double tau_ref;
 };
 
 template <typename R>
@@ -59,4 +60,8 @@ struct mfd_op {
     mfd::OpData<float> d32;
     mfd::Workspace ws;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // tiled stencil shape (TY rows per block, KZ planes per chunk); TY = 0
+    // selects the direct kernel
+    int tile64[2] = {4, 16};
+    int tile32[2] = {4, 16};
 };

@@ -33,7 +33,7 @@ class mfd_shape(C.Structure):
 
 
 class mfd_solve_opts(C.Structure):
-    _fields_ = [("tol", C.c_double), ("maxit", C.c_int64), ("precision", C.c_int32), ("jacobi", C.c_int32),
+    _fields_ = [("tol", C.c_double), ("maxit", C.c_int64), ("precision", C.c_int32), ("pad", C.c_int32),
                 ("inner_tol", C.c_double), ("check_every", C.c_int32), ("use_graph", C.c_int32)]
 
 

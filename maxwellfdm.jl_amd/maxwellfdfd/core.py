@@ -437,7 +437,7 @@ class MaxwellOperator:
         return ms.value
 
     @staticmethod
-    def _opts(tol, precision, maxit, inner_tol, jacobi, check_every, use_graph):
+    def _opts(tol, precision, maxit, inner_tol, check_every, use_graph):
         o = L.mfd_solve_opts()
         lib().mfd_solve_opts_default(C.byref(o))
         o.tol = tol
@@ -446,18 +446,17 @@ class MaxwellOperator:
             o.maxit = maxit
         if inner_tol is not None:
             o.inner_tol = inner_tol
-        o.jacobi = int(bool(jacobi))
         if check_every is not None:
             o.check_every = check_every
         o.use_graph = int(bool(use_graph))
         return o
 
-    def solve(self, b, tol=1e-8, precision="fp64", x0=None, maxit=None, inner_tol=None, jacobi=True,
+    def solve(self, b, tol=1e-8, precision="fp64", x0=None, maxit=None, inner_tol=None,
               check_every=None, use_graph=True, raise_on_noconv=True):
         b = L.carr(b)
         nrhs = b.size // self.n
         e = np.zeros_like(b) if x0 is None else L.carr(x0).copy()
-        o = self._opts(tol, precision, maxit, inner_tol, jacobi, check_every, use_graph)
+        o = self._opts(tol, precision, maxit, inner_tol, check_every, use_graph)
         st = L.mfd_solve_stats()
         rc = lib().mfd_solve(self._h, ptr(b), ptr(e), nrhs, C.byref(o), C.byref(st))
         if rc != L.MFD_OK and (raise_on_noconv or rc != L.MFD_ERR_NOCONV):
@@ -465,8 +464,8 @@ class MaxwellOperator:
         return e, st
 
     def solve_dev(self, b_ptr, e_ptr, nrhs=1, tol=1e-8, precision="fp64", maxit=None, inner_tol=None,
-                  jacobi=True, check_every=None, use_graph=True):
-        o = self._opts(tol, precision, maxit, inner_tol, jacobi, check_every, use_graph)
+                  check_every=None, use_graph=True):
+        o = self._opts(tol, precision, maxit, inner_tol, check_every, use_graph)
         st = L.mfd_solve_stats()
         check(lib().mfd_solve_dev(self._h, C.c_void_p(b_ptr), C.c_void_p(e_ptr), nrhs, C.byref(o), C.byref(st)))
         return st
@@ -474,7 +473,7 @@ class MaxwellOperator:
     def solve_adjoint(self, g, tol=1e-8, precision="fp64", maxit=None):
         g = L.carr(g)
         lam = np.zeros_like(g)
-        o = self._opts(tol, precision, maxit, None, True, None, True)
+        o = self._opts(tol, precision, maxit, None, None, True)
         st = L.mfd_solve_stats()
         check(lib().mfd_solve_adjoint(self._h, ptr(g), ptr(lam), C.byref(o), C.byref(st)))
         return lam, st

@@ -53,7 +53,8 @@ def random_boxes(N=128, npml=10, nbox=40, lam=24.0, seed=1, ndip=1):
     rng = np.random.default_rng(seed)
     lo_i, hi_i = float(npml), float(N - npml)
     for b in range(nbox):
-        s = rng.uniform(4.0, (hi_i - lo_i) / 3, size=3)
+        span = hi_i - lo_i
+        s = rng.uniform(min(4.0, span / 6), span / 3, size=3)
         lo = rng.uniform(lo_i, hi_i - s)
         eps = rng.uniform(1.0, 12.0)
         M.add_obj(m, f"box{b}", M.Cuboid(tuple(lo + s / 2), tuple(s)), ε=eps)

@@ -281,11 +281,11 @@ def test_solve_batched(pkg, O):
     op.close()
 
 
-def test_solve_without_graph_and_jacobi(pkg, O):
+def test_solve_options(pkg, O):
     m, om, b = c1_like(pkg, N=10, npml=2, r=2.0, lam=7.0)
     A = O.to_scipy(O.create_A(spec_of(m), om, m.εarr, m.μarr))
     op = pkg.MaxwellOperator(m, om)
-    for kw in (dict(use_graph=False), dict(jacobi=False), dict(check_every=5)):
+    for kw in (dict(use_graph=False), dict(check_every=5), dict(precision="mixed", inner_tol=1e-3)):
         say("solve options", kw)
         e, st = op.solve(b, tol=1e-8, **kw)
         assert np.linalg.norm(b - A @ e) / np.linalg.norm(b) <= 1.05e-8
