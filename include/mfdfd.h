@@ -37,6 +37,7 @@ enum {
 enum { MFD_EE = 0, MFD_HH = 1 };      /* FieldType */
 enum { MFD_PRIM = 0, MFD_DUAL = 1 };  /* GridType */
 enum { MFD_FP64 = 0, MFD_MIXED = 1 }; /* solve precision */
+enum { MFD_SQMR = 0, MFD_COCG = 1 };  /* Krylov method (MFD_COCG: with reliable updates) */
 
 typedef struct { double re, im; } mfd_c64;
 
@@ -71,7 +72,7 @@ typedef struct {
     double tol;               /* relative residual ||b-Ae||/||b|| in FP64 (1e-8) */
     int64_t maxit;            /* total Krylov iterations allowed per RHS */
     int32_t precision;        /* MFD_FP64 or MFD_MIXED (FP32 inner, FP64 refinement) */
-    int32_t pad;
+    int32_t method;           /* MFD_SQMR, MFD_COCG, or -1: SQMR for FP64, COCG for mixed */
     double inner_tol;         /* mixed: tightest inner reduction per FP32 cycle (1e-5) */
     int32_t check_every;      /* iterations between host convergence checks (32) */
     int32_t use_graph;        /* capture a chunk of iterations in a hipGraph */
@@ -79,7 +80,7 @@ typedef struct {
 
 typedef struct {
     int64_t iterations;       /* Krylov iterations (max over RHS) */
-    int64_t outer_cycles;     /* refinement cycles (mixed) */
+    int64_t outer_cycles;     /* refinement cycles (SQMR) / reliable updates (COCG) */
     double relres;            /* max over RHS of the final FP64 relative residual */
     double seconds;           /* device time of the solve */
     int32_t converged;
@@ -142,7 +143,22 @@ int mfd_op_destroy(mfd_op *op);
 int mfd_op_apply(mfd_op *op, const mfd_c64 *x, mfd_c64 *y, int64_t nrhs);
 /* device pointers; prec MFD_FP64 (complex double) or MFD_MIXED (complex float) */
 int mfd_op_apply_dev(mfd_op *op, const void *x, void *y, int64_t nrhs, int prec);
-/* Average device time (ms) of the last n applies of the dominant stencil kernel */
+/* Inner operator of the solver: the grad-div augmented A' = A − ε G W Div ε
+ * (W = 1/ε̄² at the primal nodes) with right-hand side r + ω⁻² ε G W Div r.
+ * Div Cₘ = 0 makes every solution of A e = b a solution of the augmented
+ * system, so the discretisation and the FP64 residual checked are unchanged;
+ * the longitudinal near-null space of Cₘ Cₑ is moved away from zero.  On by
+ * default where eligible (diagonal ε, μ = 1, ω ≠ 0).  No reference
+ * counterpart: the reference has no solver (model.jl stops at create_A). */
+int mfd_op_set_graddiv(mfd_op *op, int enable);
+/* Tile-shape autotuner for the stencil (rows per block TY, z-planes per block
+ * KZ): times each candidate for nrhs right-hand sides and keeps the fastest
+ * per precision.  Outputs may be null.  No reference counterpart. */
+int mfd_op_autotune(mfd_op *op, int64_t nrhs, int iters, int tile64[2], int tile32[2], double ms[2]);
+/* Fix the tile shape by hand: TY in {1,2,4} (0 = cache-reliant direct kernel), KZ >= 1. */
+int mfd_op_set_tile(mfd_op *op, int prec, int ty, int kz);
+/* Average device time (ms) of one apply of the solver's stencil kernel
+ * (MFD_FP64: A; MFD_MIXED: the complex-FP32 inner operator) */
 int mfd_op_bench_apply(mfd_op *op, int64_t nrhs, int prec, int iters, double *ms_per_apply);
 
 /* Solve A e = b (the user's `A \ b`); e holds the initial guess on entry. */

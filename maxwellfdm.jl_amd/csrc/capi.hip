@@ -10,6 +10,9 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
 int op_destroy_impl(mfd_op *op);
 int op_apply_impl(mfd_op *op, const mfd_c64 *x, mfd_c64 *y, int64_t nrhs);
 int op_apply_dev_impl(mfd_op *op, const void *x, void *y, int64_t nrhs, int prec);
+int op_set_graddiv_impl(mfd_op *op, int enable);
+int op_set_tile_impl(mfd_op *op, int prec, int ty, int kz);
+int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best32, double *ms_out);
 int op_bench_apply_impl(mfd_op *op, int64_t nrhs, int prec, int iters, double *ms);
 void solve_opts_default(mfd_solve_opts *o);
 int solve_impl(mfd_op *op, const mfd_c64 *b, mfd_c64 *e, int64_t nrhs, const mfd_solve_opts *o, mfd_solve_stats *st);
@@ -233,6 +236,20 @@ int mfd_op_apply_dev(mfd_op *op, const void *x, void *y, int64_t nrhs, int prec)
     if (!op) { set_error("null operator"); return MFD_ERR_ARG; }
     CHECK_CTX(op->ctx);
     return op_apply_dev_impl(op, x, y, nrhs, prec);
+}
+int mfd_op_set_graddiv(mfd_op *op, int enable) {
+    if (!op) { set_error("null operator"); return MFD_ERR_ARG; }
+    CHECK_CTX(op->ctx);
+    return op_set_graddiv_impl(op, enable);
+}
+int mfd_op_set_tile(mfd_op *op, int prec, int ty, int kz) {
+    if (!op) { set_error("null operator"); return MFD_ERR_ARG; }
+    return op_set_tile_impl(op, prec, ty, kz);
+}
+int mfd_op_autotune(mfd_op *op, int64_t nrhs, int iters, int tile64[2], int tile32[2], double ms[2]) {
+    if (!op || nrhs < 1 || iters < 1) { set_error("bad argument"); return MFD_ERR_ARG; }
+    CHECK_CTX(op->ctx);
+    return op_autotune_impl(op, nrhs, iters, tile64, tile32, ms);
 }
 int mfd_op_bench_apply(mfd_op *op, int64_t nrhs, int prec, int iters, double *ms) {
     if (!op || !ms || iters < 1) { set_error("bad argument"); return MFD_ERR_ARG; }

@@ -206,11 +206,13 @@ __device__ inline bool reduce_last(double (&v)[NV], double *part, unsigned *cnt,
 template <typename R, int MODE, bool OFFD, bool MU, int EPI>
 __global__ __launch_bounds__(BS) void k_apply(DGrid<R> g, OpData<R> od, const cx<R> *__restrict__ xall,
                                               cx<R> *__restrict__ yall, const c64 *__restrict__ ball, int64_t vs,
-                                              double *part, unsigned *cnt, Scal *scal, double *rn) {
+                                              double *part, unsigned *cnt, Scal *scal, double *rn,
+                                              const int *__restrict__ mask = nullptr) {
     const int rhs = blockIdx.y;
     const cx<R> *x = xall + rhs * vs;
     cx<R> *y = yall + rhs * vs;
     bool skip = false;
+    if (EPI == EPI_RESID && mask && !mask[rhs]) return;
     if (EPI == EPI_DOT) {
         if (scal[rhs].done) return;
         skip = scal[rhs].conv != 0;
@@ -331,11 +333,11 @@ __global__ __launch_bounds__(BS) void k_upd_r(cx<R> *__restrict__ rall, const cx
     if (S.done || S.conv) return;
     cx<R> *r = rall + rhs * vs;
     const cx<R> *t = tall + rhs * vs;
-    cx<R> al = mk<R>((R)S.alpha_re, (R)S.alpha_im);
+    c64 al = mk<double>(S.alpha_re, S.alpha_im);
     double acc[3] = {0, 0, 0};
     for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
-        cx<R> v = r[e] - al * t[e];
-        r[e] = v;
+        c64 v = cvt<double>(r[e]) - al * cvt<double>(t[e]);
+        r[e] = cvt<R>(v);
         acc[0] += (double)v.re * v.re + (double)v.im * v.im;
         acc[1] += (double)v.re * v.re - (double)v.im * v.im;
         acc[2] += 2.0 * (double)v.re * v.im;
@@ -362,15 +364,6 @@ __global__ __launch_bounds__(BS) void k_upd_r(cx<R> *__restrict__ rall, const cx
     S.rnorm2 = tot[0];
     S.it += 1;
     if (S.tau <= S.target || S.it >= S.itmax || rn == 0.0) S.conv = 1;
-    // FP32 inner iterations: once the recurrence stagnates, τ only crawls;
-    // end the cycle when a window of iterations gains less than 2x
-    if (sizeof(R) == 4) {
-        if (S.it - S.it_ref >= 400) {
-            if (S.tau > 0.5 * S.tau_ref) { S.conv = 1; S.stag = 1; }
-            S.tau_ref = S.tau;
-            S.it_ref = S.it;
-        }
-    }
 }
 
 // d = c²θ² d + c²α q;  z += d;  q = r + β q
@@ -383,20 +376,20 @@ __global__ __launch_bounds__(BS) void k_upd_dzq(cx<R> *__restrict__ dall, cx<R> 
     if (S.done) return;
     cx<R> *d = dall + rhs * vs, *z = zall + rhs * vs, *q = qall + rhs * vs;
     const cx<R> *r = rall + rhs * vs;
-    R c2t2 = (R)S.c2t2;
-    cx<R> c2a = mk<R>((R)S.c2a_re, (R)S.c2a_im), be = mk<R>((R)S.beta_re, (R)S.beta_im);
+    double c2t2 = S.c2t2;
+    c64 c2a = mk<double>(S.c2a_re, S.c2a_im), be = mk<double>(S.beta_re, S.beta_im);
     for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
-        cx<R> qe = q[e];
-        cx<R> de = c2t2 * d[e] + c2a * qe;
-        d[e] = de;
-        z[e] = z[e] + de;
-        q[e] = r[e] + be * qe;
+        c64 qe = cvt<double>(q[e]);
+        c64 de = c2t2 * cvt<double>(d[e]) + c2a * qe;
+        d[e] = cvt<R>(de);
+        z[e] = cvt<R>(cvt<double>(z[e]) + de);
+        q[e] = cvt<R>(cvt<double>(r[e]) + be * qe);
     }
 }
 
 // cycle start: c = (Q T) r64 in precision R; r = q = c; d = z = 0
 template <typename R>
-__global__ __launch_bounds__(BS) void k_init(DGrid<R> g, OpData<R> od, const c64 *__restrict__ r64all,
+__global__ __launch_bounds__(BS) void k_init(DGrid<double> g, OpData<double> od, const c64 *__restrict__ r64all,
                                              cx<R> *rall, cx<R> *qall, cx<R> *dall, cx<R> *zall, int64_t n,
                                              int64_t vs, double *part, unsigned *cnt, Scal *scal, double rel_target,
                                              const int *active, long long itmax) {
@@ -413,7 +406,7 @@ __global__ __launch_bounds__(BS) void k_init(DGrid<R> g, OpData<R> od, const c64
         cell3(g, p, i, j, k);
         cx<R> v = czero<R>();
         if (act) {
-            c64 l = cvt<double>(od.q[3 * p + c] * tfac(g, c, i, j, k, false));
+            c64 l = cvt<double>(od.q[3 * p + c]) * tfac(g, c, i, j, k, false);
             c64 s = l * r64[e];
             v = mk<R>((R)s.re, (R)s.im);
         }
@@ -443,7 +436,7 @@ __global__ __launch_bounds__(BS) void k_init(DGrid<R> g, OpData<R> od, const c64
 
 // e64 += (Q T⁻¹) z
 template <typename R>
-__global__ __launch_bounds__(BS) void k_upd_e(DGrid<R> g, OpData<R> od, c64 *__restrict__ eall,
+__global__ __launch_bounds__(BS) void k_upd_e(DGrid<double> g, OpData<double> od, c64 *__restrict__ eall,
                                               const cx<R> *__restrict__ zall, int64_t n, int64_t vs) {
     const int rhs = blockIdx.y;
     c64 *e64 = eall + rhs * vs;
@@ -456,8 +449,29 @@ __global__ __launch_bounds__(BS) void k_upd_e(DGrid<R> g, OpData<R> od, c64 *__r
         dof2cp(g, e, c, p);
         int i, j, k;
         cell3(g, p, i, j, k);
-        c64 rf = cvt<double>(od.q[3 * p + c] * tfac(g, c, i, j, k, true));
+        c64 rf = cvt<double>(od.q[3 * p + c]) * tfac(g, c, i, j, k, true);
         e64[e] = e64[e] + rf * cvt<double>(ze);
+    }
+}
+
+
+// probe of the true residual inside a cycle: et = e + (Q T⁻¹) z (FP64)
+template <typename R>
+__global__ __launch_bounds__(BS) void k_probe_x(DGrid<double> g, OpData<double> od, const c64 *__restrict__ eall,
+                                                const cx<R> *__restrict__ zall, c64 *__restrict__ etall, int64_t n,
+                                                int64_t vs) {
+    const int rhs = blockIdx.y;
+    const c64 *e64 = eall + rhs * vs;
+    const cx<R> *z = zall + rhs * vs;
+    c64 *et = etall + rhs * vs;
+    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+        int c;
+        int64_t p;
+        dof2cp(g, e, c, p);
+        int i, j, k;
+        cell3(g, p, i, j, k);
+        c64 rf = cvt<double>(od.q[3 * p + c]) * tfac(g, c, i, j, k, true);
+        et[e] = e64[e] + rf * cvt<double>(z[e]);
     }
 }
 
@@ -531,6 +545,98 @@ __global__ void k_jacobi(DGrid<double> g, OpData<double> od, c64 *q, int use) {
     }
 }
 
+
+// grad-div weight W = 1/ε̄² at each primal node, ε̄ the mean of the ε_ww of
+// the (up to 6) E edges meeting at the node
+__global__ void k_wnode(DGrid<double> g, const c64 *epsd, c64 *w, c32 *w32, int wmode) {
+    int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= g.M) return;
+    int id[3];
+    cell3(g, p, id[0], id[1], id[2]);
+    c64 s = mk<double>(0, 0);
+    int n = 0;
+    for (int a = 0; a < 3; ++a) {
+        s = s + epsd[3 * p + a];
+        ++n;
+        int64_t st = a == 0 ? 1 : (a == 1 ? (int64_t)g.N[0] : (int64_t)g.N[0] * g.N[1]);
+        if (id[a] > 0) { s = s + epsd[3 * (p - st) + a]; ++n; }
+        else if (g.bloch[a]) { s = s + epsd[3 * (p + (int64_t)(g.N[a] - 1) * st) + a]; ++n; }
+    }
+    c64 m = mk<double>(s.re / n, s.im / n);
+    c64 v = wmode == 1 ? cinv(m) : (wmode == 2 ? mk<double>(1, 0) : cinv(m * m));
+    w[p] = v;
+    w32[p] = mk<float>((float)v.re, (float)v.im);
+}
+
+// Jacobi scaling of the grad-div form: q = (c − ω² ε + ε² (G W Div)-diagonal)^{-1/2}
+__global__ void k_jacobi_gd(DGrid<double> g, OpData<double> od, c64 *q) {
+    int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= g.M) return;
+    int id[3];
+    cell3(g, p, id[0], id[1], id[2]);
+    for (int u = 0; u < 3; ++u) {
+        c64 e = od.epsd[3 * p + u];
+        c64 dg = ccdiag<double, false>(g, od, u, id[0], id[1], id[2]) - od.om2 * e;
+        int Nu = g.N[u], iu = id[u];
+        int64_t st = u == 0 ? 1 : (u == 1 ? (int64_t)g.N[0] : (int64_t)g.N[0] * g.N[1]);
+        c64 invD = g.sdlinv[DUAL][u][iu];
+        c64 gd;
+        if (Nu == 1 && g.bloch[u]) {
+            c64 f = (g.ph[u] - mk<double>(1, 0)) * (mk<double>(1, 0) - g.phinv[u]);
+            gd = -((invD * g.sdlinv[PRIM][u][0]) * (od.wnode[p] * f));
+        } else {
+            gd = od.wnode[p] * g.sdlinv[PRIM][u][iu];
+            if (iu < Nu - 1) gd = gd + od.wnode[p + st] * g.sdlinv[PRIM][u][iu + 1];
+            else if (g.bloch[u]) gd = gd + od.wnode[p - (int64_t)(Nu - 1) * st] * g.sdlinv[PRIM][u][0];
+            gd = invD * gd;
+        }
+        dg = dg + (e * e) * gd;
+        q[3 * p + u] = cinv(csqrt_(dg));
+    }
+}
+
+// ψ = W Div(r) at the primal nodes (backward differences), per RHS
+__global__ __launch_bounds__(BS) void k_gd_psi(DGrid<double> g, const c64 *wnode, const c64 *rall, c64 *psiall,
+                                               int64_t vs) {
+    const int rhs = blockIdx.y;
+    const c64 *r = rall + rhs * vs;
+    c64 *psi = psiall + rhs * g.M;
+    for (int64_t p = blockIdx.x * (int64_t)BS + threadIdx.x; p < g.M; p += (int64_t)gridDim.x * BS) {
+        int id[3];
+        cell3(g, p, id[0], id[1], id[2]);
+        c64 dv = mk<double>(0, 0);
+        for (int w = 0; w < 3; ++w) {
+            int64_t st = w == 0 ? 1 : (w == 1 ? (int64_t)g.N[0] : (int64_t)g.N[0] * g.N[1]);
+            c64 a = r[dofidx(g, w, p)], b = mk<double>(0, 0);
+            if (id[w] > 0) b = r[dofidx(g, w, p - st)];
+            else if (g.bloch[w]) b = r[dofidx(g, w, p + (int64_t)(g.N[w] - 1) * st)] * g.phinv[w];
+            dv = dv + g.sdlinv[PRIM][w][id[w]] * (a - b);
+        }
+        psi[p] = wnode[p] * dv;
+    }
+}
+
+// r' = r + (1/ω²) ε G ψ (forward differences of ψ to the E edges), in place
+__global__ __launch_bounds__(BS) void k_gd_rhs(DGrid<double> g, OpData<double> od, const c64 *psiall, c64 *rall,
+                                               int64_t vs, c64 inv_om2) {
+    const int rhs = blockIdx.y;
+    c64 *r = rall + rhs * vs;
+    const c64 *psi = psiall + rhs * g.M;
+    for (int64_t p = blockIdx.x * (int64_t)BS + threadIdx.x; p < g.M; p += (int64_t)gridDim.x * BS) {
+        int id[3];
+        cell3(g, p, id[0], id[1], id[2]);
+        for (int u = 0; u < 3; ++u) {
+            int64_t st = u == 0 ? 1 : (u == 1 ? (int64_t)g.N[0] : (int64_t)g.N[0] * g.N[1]);
+            c64 pn = mk<double>(0, 0);
+            if (id[u] < g.N[u] - 1) pn = psi[p + st];
+            else if (g.bloch[u]) pn = psi[p - (int64_t)(g.N[u] - 1) * st] * g.ph[u];
+            c64 gv = g.sdlinv[DUAL][u][id[u]] * (pn - psi[p]);
+            int64_t e = dofidx(g, u, p);
+            r[e] = r[e] + inv_om2 * (od.epsd[3 * p + u] * gv);
+        }
+    }
+}
+
 __global__ void k_down(const c64 *a, c32 *b, int64_t n) {
     int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (e < n) b[e] = mk<float>((float)a[e].re, (float)a[e].im);
@@ -567,9 +673,10 @@ static int ws_ensure(mfd_op *op, int64_t nrhs, int prec) {
     size_t vb = es * n * nrhs;
     size_t align = 256;
     auto up = [&](size_t x) { return (x + align - 1) / align * align; };
-    size_t bytes = 5 * up(vb) + up(sizeof(c64) * n * nrhs) + up(sizeof(double) * 3 * nrhs * NPART * 4) +
+    size_t bytes = 5 * up(vb) + 3 * up(sizeof(c64) * n * nrhs) + up(sizeof(c64) * (n / 3) * nrhs) +
+                   up(sizeof(double) * 3 * nrhs * NPART * 4) +
                    up(sizeof(unsigned) * 3 * nrhs) + up(sizeof(Scal) * nrhs) + 2 * up(sizeof(double) * nrhs) +
-                   up(sizeof(int) * nrhs);
+                   up(sizeof(int) * 3 * nrhs);
     MFD_HIP(hipMalloc(&w.buf, bytes));
     char *p = (char *)w.buf;
     w.r = p; p += up(vb);
@@ -578,12 +685,15 @@ static int ws_ensure(mfd_op *op, int64_t nrhs, int prec) {
     w.d = p; p += up(vb);
     w.z = p; p += up(vb);
     w.r64 = p; p += up(sizeof(c64) * n * nrhs);
+    w.et = p; p += up(sizeof(c64) * n * nrhs);
+    w.et2 = p; p += up(sizeof(c64) * n * nrhs);
+    w.psi = p; p += up(sizeof(c64) * (n / 3) * nrhs);
     w.part = (double *)p; p += up(sizeof(double) * 3 * nrhs * NPART * 4);
     w.cnt = (unsigned *)p; p += up(sizeof(unsigned) * 3 * nrhs);
     w.scal = (Scal *)p; p += up(sizeof(Scal) * nrhs);
     w.bn = (double *)p; p += up(sizeof(double) * nrhs);
     w.rn = (double *)p; p += up(sizeof(double) * nrhs);
-    w.active = (int *)p; p += up(sizeof(int) * nrhs);
+    w.active = (int *)p; p += up(sizeof(int) * 3 * nrhs);
     MFD_HIP(hipMemsetAsync(w.cnt, 0, sizeof(unsigned) * 3 * nrhs, op->ctx->stream));
     MFD_HIP(hipHostMalloc((void **)&w.scal_h, sizeof(Scal) * nrhs, hipHostMallocDefault));
     MFD_HIP(hipHostMalloc((void **)&w.rn_h, sizeof(double) * nrhs, hipHostMallocDefault));
@@ -602,7 +712,7 @@ template <> inline const OpData<float> &opd<float>(const mfd_op *op) { return op
 // launch one operator apply
 template <typename R, int MODE, int EPI>
 static void launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int64_t nrhs, double *part, unsigned *cnt,
-                         Scal *scal, double *rn) {
+                         Scal *scal, double *rn, const int *mask = nullptr) {
     const DGrid<R> &g = op->G.view<R>();
     const OpData<R> &od = opd<R>(op);
     int64_t vs = 3 * g.M;
@@ -614,18 +724,24 @@ static void launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int
         int nch = (g.N[2] + kz - 1) / kz;
         while ((int64_t)nx * ny * nch > NPART) { kz *= 2; nch = (g.N[2] + kz - 1) / kz; }
         dim3 grid(nx, ny, (unsigned)(nch * nrhs));
-        if (ty == 2) k_apply_tiled<R, MODE, EPI, 2><<<grid, 128, 0, s>>>(g, od, x, y, b, vs, kz, nch, part, cnt, scal, rn);
-        else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4><<<grid, 256, 0, s>>>(g, od, x, y, b, vs, kz, nch, part, cnt, scal, rn);
-        else k_apply_tiled<R, MODE, EPI, 1><<<grid, 64, 0, s>>>(g, od, x, y, b, vs, kz, nch, part, cnt, scal, rn);
+        if (MODE == MODE_P && op->gd) {
+            if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, true><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, cnt, scal, rn, mask);
+            else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, true><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, cnt, scal, rn, mask);
+            else k_apply_tiled<R, MODE, EPI, 1, true><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, cnt, scal, rn, mask);
+        } else {
+            if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, false><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, cnt, scal, rn, mask);
+            else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, false><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, cnt, scal, rn, mask);
+            else k_apply_tiled<R, MODE, EPI, 1, false><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, cnt, scal, rn, mask);
+        }
         return;
     }
     dim3 grid(ngrid(g.M), (unsigned)nrhs);
     if (op->full) {
-        if (op->has_mu) k_apply<R, MODE, true, true, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, cnt, scal, rn);
-        else k_apply<R, MODE, true, false, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, cnt, scal, rn);
+        if (op->has_mu) k_apply<R, MODE, true, true, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, cnt, scal, rn, mask);
+        else k_apply<R, MODE, true, false, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, cnt, scal, rn, mask);
     } else {
-        if (op->has_mu) k_apply<R, MODE, false, true, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, cnt, scal, rn);
-        else k_apply<R, MODE, false, false, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, cnt, scal, rn);
+        if (op->has_mu) k_apply<R, MODE, false, true, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, cnt, scal, rn, mask);
+        else k_apply<R, MODE, false, false, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, cnt, scal, rn, mask);
     }
 }
 
@@ -681,7 +797,7 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
     // device arrays: epsd, epso, muinv, q (FP64) and their FP32 copies, + asym flag
     size_t n3 = 3 * (size_t)M;
     size_t b64 = sizeof(c64) * n3, b32 = sizeof(c32) * n3;
-    size_t bytes = 4 * (b64 + b32) + 256;
+    size_t bytes = 4 * (b64 + b32) + (b64 + b32) / 3 + 512;
     if (hipMalloc(&op->buf, bytes) != hipSuccess) { set_error("hipMalloc failed for operator arrays"); return bail(MFD_ERR_ALLOC); }
     char *p = (char *)op->buf;
     c64 *epsd = (c64 *)p; p += b64;
@@ -692,6 +808,9 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
     c32 *epso32 = (c32 *)p; p += b32;
     c32 *muinv32 = (c32 *)p; p += b32;
     c32 *q32 = (c32 *)p; p += b32;
+    c64 *wn = (c64 *)p; p += b64 / 3;
+    c32 *wn32 = (c32 *)p; p += b32 / 3;
+    p = (char *)(((uintptr_t)p + 63) & ~(uintptr_t)63);
     int *asym = (int *)p;
     if (hipMemsetAsync(asym, 0, sizeof(int), s) != hipSuccess) return bail(MFD_ERR_HIP);
     k_pack<<<nb(M), BS, 0, s>>>((const c64 *)E.p, (const c64 *)U.p, M, epsd, full ? epso : nullptr,
@@ -705,7 +824,13 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
     op->d64.muinv = op->has_mu ? muinv : nullptr;
     op->d64.q = q;
     op->d64.om2 = om2;
-    if (op->has_mu) k_jacobi<true><<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q, 1);
+    op->d64.wnode = wn;
+    op->gd_ok = !full && !op->has_mu && !(om2.re == 0.0 && om2.im == 0.0);
+    op->gd = 0;   // opt-in: it helps small PML-dominated grids and slows larger ones
+    const char *wm = getenv("MFD_GD_WMODE");
+    k_wnode<<<nb(M), BS, 0, s>>>(op->G.d64, epsd, wn, wn32, wm ? atoi(wm) : 0);
+    if (op->gd) k_jacobi_gd<<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q);
+    else if (op->has_mu) k_jacobi<true><<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q, 1);
     else k_jacobi<false><<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q, 1);
     k_down<<<nb(n3), BS, 0, s>>>(epsd, epsd32, n3);
     if (full) k_down<<<nb(n3), BS, 0, s>>>(epso, epso32, n3);
@@ -715,6 +840,7 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
     op->d32.epso = full ? epso32 : nullptr;
     op->d32.muinv = op->has_mu ? muinv32 : nullptr;
     op->d32.q = q32;
+    op->d32.wnode = wn32;
     op->d32.om2 = mk<float>((float)om2.re, (float)om2.im);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) { set_error("operator set-up failed"); return bail(MFD_ERR_HIP); }
     hipEventCreate(&op->ev0);
@@ -731,6 +857,25 @@ int op_destroy_impl(mfd_op *op) {
     if (op->ev0) hipEventDestroy(op->ev0);
     if (op->ev1) hipEventDestroy(op->ev1);
     delete op;
+    return MFD_OK;
+}
+
+int op_set_graddiv_impl(mfd_op *op, int enable) {
+    if (enable && !op->gd_ok) {
+        set_error("grad-div form needs diagonal ε, μ = 1 and ω ≠ 0");
+        return MFD_ERR_UNSUPPORTED;
+    }
+    op->gd = enable ? 1 : 0;
+    hipStream_t s = op->ctx->stream;
+    int64_t M = op->G.h.M;
+    c64 *q = (c64 *)op->d64.q;
+    if (op->gd) k_jacobi_gd<<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q);
+    else if (op->has_mu) k_jacobi<true><<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q, 1);
+    else k_jacobi<false><<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q, 1);
+    k_down<<<nb(3 * M), BS, 0, s>>>(q, (c32 *)op->d32.q, 3 * M);
+    op->ws.release();   // captured graphs hold the old kernels
+    MFD_HIP(hipGetLastError());
+    MFD_HIP(hipStreamSynchronize(s));
     return MFD_OK;
 }
 
@@ -786,19 +931,194 @@ int op_bench_apply_impl(mfd_op *op, int64_t nrhs, int prec, int iters, double *m
     return MFD_OK;
 }
 
-void solve_opts_default(mfd_solve_opts *o) {
-    o->tol = 1e-8;
-    o->maxit = 200000;
-    o->precision = MFD_FP64;
-    o->pad = 0;
-    o->inner_tol = 1e-5;
-    o->check_every = 32;
-    o->use_graph = 1;
+// Tile-shape autotuner: times the solver's stencils (FP64 A with the residual
+// epilogue shape, complex-FP32 inner operator) for each candidate (TY, KZ) and
+// keeps the fastest per precision.
+int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best32, double *ms_out) {
+    if (op->full || op->has_mu) {   // the direct kernel serves these operators
+        op->tile64[0] = op->tile32[0] = 0;
+        if (best64) best64[0] = best64[1] = 0;
+        if (best32) best32[0] = best32[1] = 0;
+        return MFD_OK;
+    }
+    const int tys[3] = {1, 2, 4}, kzs[4] = {4, 8, 16, 32};
+    double bestms[2] = {1e30, 1e30};
+    int bt[2][2] = {{4, 16}, {4, 16}};
+    for (int prec = 0; prec < 2; ++prec)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 4; ++b) {
+                int *tile = prec == 0 ? op->tile64 : op->tile32;
+                tile[0] = tys[a];
+                tile[1] = kzs[b];
+                double ms = 0;
+                MFD_TRY(op_bench_apply_impl(op, nrhs, prec == 0 ? MFD_FP64 : MFD_MIXED, iters, &ms));
+                if (ms < bestms[prec]) { bestms[prec] = ms; bt[prec][0] = tys[a]; bt[prec][1] = kzs[b]; }
+            }
+    op->tile64[0] = bt[0][0]; op->tile64[1] = bt[0][1];
+    op->tile32[0] = bt[1][0]; op->tile32[1] = bt[1][1];
+    op->ws.release();   // captured graphs hold the old launch shapes
+    if (best64) { best64[0] = bt[0][0]; best64[1] = bt[0][1]; }
+    if (best32) { best32[0] = bt[1][0]; best32[1] = bt[1][1]; }
+    if (ms_out) { ms_out[0] = bestms[0]; ms_out[1] = bestms[1]; }
+    return MFD_OK;
+}
+
+
+// ------------------------------------------------ COCG with reliable updates
+//
+// Conjugate orthogonal CG (van der Vorst & Melissen 1990) on the complex
+// symmetric P, iterating on vectors of precision R with FP64 reductions.
+// Reliable updates (Sleijpen & van der Vorst 1996; as in mixed-precision CG,
+// Clark et al. 2010): when the recursive residual has fallen by δ below its
+// running maximum, the iterate x is flushed into the FP64 solution e, the
+// true residual b − A e is recomputed with the FP64 operator and replaces r;
+// the search direction p is kept.  Convergence is decided on that FP64
+// residual only.
+
+// x += α p;  r −= α t;  reduce ||r||², rᵀr;  β, ρ, running max in the last block
+template <typename R>
+__global__ __launch_bounds__(BS) void k_cocg_xr(cx<R> *__restrict__ xall, cx<R> *__restrict__ rall,
+                                                const cx<R> *__restrict__ pall, const cx<R> *__restrict__ tall,
+                                                int64_t n, int64_t vs, double *part, unsigned *cnt, Scal *scal) {
+    const int rhs = blockIdx.y;
+    Scal &S = scal[rhs];
+    if (S.done || S.breakdown) return;
+    cx<R> *x = xall + rhs * vs, *r = rall + rhs * vs;
+    const cx<R> *p = pall + rhs * vs, *t = tall + rhs * vs;
+    c64 al = mk<double>(S.alpha_re, S.alpha_im);
+    double acc[3] = {0, 0, 0};
+    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+        x[e] = cvt<R>(cvt<double>(x[e]) + al * cvt<double>(p[e]));
+        c64 v = cvt<double>(r[e]) - al * cvt<double>(t[e]);
+        r[e] = cvt<R>(v);
+        acc[0] += (double)v.re * v.re + (double)v.im * v.im;
+        acc[1] += (double)v.re * v.re - (double)v.im * v.im;
+        acc[2] += 2.0 * (double)v.re * v.im;
+    }
+    double tot[3];
+    if (!reduce_last<3>(acc, part + (size_t)rhs * NPART * 4, cnt + rhs, tot)) return;
+    if (threadIdx.x != 0) return;
+    c64 rho1 = mk<double>(tot[1], tot[2]);
+    if (S.rho_re == 0.0 && S.rho_im == 0.0) { S.breakdown = 1; S.beta_re = S.beta_im = 0; }
+    else {
+        c64 b = cdiv(rho1, mk<double>(S.rho_re, S.rho_im));
+        S.beta_re = b.re;
+        S.beta_im = b.im;
+    }
+    S.rho_re = rho1.re;
+    S.rho_im = rho1.im;
+    S.rnorm2 = tot[0];
+    double rn = sqrt(tot[0]);
+    if (rn > S.tau_ref) S.tau_ref = rn;   // running maximum since the last update
+    S.it += 1;
+}
+
+// p = r + β p
+template <typename R>
+__global__ __launch_bounds__(BS) void k_cocg_p(cx<R> *__restrict__ pall, const cx<R> *__restrict__ rall, int64_t n,
+                                               int64_t vs, const Scal *scal) {
+    const int rhs = blockIdx.y;
+    const Scal &S = scal[rhs];
+    if (S.done || S.breakdown) return;
+    cx<R> *p = pall + rhs * vs;
+    const cx<R> *r = rall + rhs * vs;
+    c64 be = mk<double>(S.beta_re, S.beta_im);
+    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS)
+        p[e] = cvt<R>(cvt<double>(r[e]) + be * cvt<double>(p[e]));
+}
+
+// e64 += (Q T⁻¹) x;  x = 0   (masked RHS)
+template <typename R>
+__global__ __launch_bounds__(BS) void k_ru_flush(DGrid<double> g, OpData<double> od, c64 *__restrict__ eall,
+                                                 cx<R> *__restrict__ xall, int64_t n, int64_t vs, const int *mask) {
+    const int rhs = blockIdx.y;
+    if (!mask[rhs]) return;
+    c64 *e64 = eall + rhs * vs;
+    cx<R> *x = xall + rhs * vs;
+    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+        cx<R> xe = x[e];
+        if (xe.re == 0 && xe.im == 0) continue;
+        int c;
+        int64_t p;
+        dof2cp(g, e, c, p);
+        int i, j, k;
+        cell3(g, p, i, j, k);
+        c64 rf = cvt<double>(od.q[3 * p + c]) * tfac(g, c, i, j, k, true);
+        e64[e] = e64[e] + rf * cvt<double>(xe);
+        x[e] = czero<R>();
+    }
+}
+
+// r = (Q T) r64 in precision R (p = r as well on a restart); ρ = rᵀr, running
+// max and scale reset (masked RHS)
+template <typename R>
+__global__ __launch_bounds__(BS) void k_ru_replace(DGrid<double> g, OpData<double> od, const c64 *__restrict__ r64all,
+                                                   cx<R> *__restrict__ rall, cx<R> *__restrict__ pall, int64_t n,
+                                                   int64_t vs, double *part, unsigned *cnt, Scal *scal,
+                                                   const int *mask, const int *restart, const double *rn_true) {
+    const int rhs = blockIdx.y;
+    if (!mask[rhs]) return;
+    const c64 *r64 = r64all + rhs * vs;
+    cx<R> *r = rall + rhs * vs, *p = pall + rhs * vs;
+    const bool rs = restart[rhs] != 0;
+    double acc[3] = {0, 0, 0};
+    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+        int c;
+        int64_t pc;
+        dof2cp(g, e, c, pc);
+        int i, j, k;
+        cell3(g, pc, i, j, k);
+        c64 l = cvt<double>(od.q[3 * pc + c]) * tfac(g, c, i, j, k, false);
+        c64 sv = l * r64[e];
+        cx<R> v = mk<R>((R)sv.re, (R)sv.im);
+        r[e] = v;
+        if (rs) p[e] = v;
+        acc[0] += (double)v.re * v.re + (double)v.im * v.im;
+        acc[1] += (double)v.re * v.re - (double)v.im * v.im;
+        acc[2] += 2.0 * (double)v.re * v.im;
+    }
+    double tot[3];
+    if (!reduce_last<3>(acc, part + (size_t)rhs * NPART * 4, cnt + rhs, tot)) return;
+    if (threadIdx.x != 0) return;
+    Scal &S = scal[rhs];
+    S.rho_re = tot[1];
+    S.rho_im = tot[2];
+    S.rnorm2 = tot[0];
+    S.cn = sqrt(tot[0]);       // scaled norm of the replaced residual
+    S.tau_ref = S.cn;          // running maximum restarts here
+    S.tau = sqrt(rn_true[rhs]);   // its true (unscaled) norm
+    S.breakdown = 0;
+    if (S.cn == 0.0) S.done = 1;
+}
+
+__global__ void k_set_done(Scal *scal, const int *done, int64_t nrhs) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrhs && done[r]) scal[r].done = 1;
+}
+
+__global__ void k_scal_reset(Scal *scal, int64_t nrhs) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrhs) memset(&scal[r], 0, sizeof(Scal));
 }
 
 template <typename R>
-static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double rel_target, const int *active_d,
-                     long long budget, long long *iters_out) {
+static void launch_cocg_iter(mfd_op *op, int64_t nrhs) {
+    Workspace &w = op->ws;
+    int64_t n = w.n;
+    hipStream_t s = op->ctx->stream;
+    dim3 gv(ngrid(n), (unsigned)nrhs);
+    size_t pslot = (size_t)nrhs * NPART * 4;
+    // w.q holds p, w.t holds t = P p, w.z holds x, w.r holds r
+    launch_apply<R, MODE_P, EPI_DOT>(op, (const cx<R> *)w.q, (cx<R> *)w.t, nullptr, nrhs, w.part, w.cnt, w.scal, nullptr);
+    k_cocg_xr<R><<<gv, BS, 0, s>>>((cx<R> *)w.z, (cx<R> *)w.r, (const cx<R> *)w.q, (const cx<R> *)w.t, n, n,
+                                   w.part + pslot, w.cnt + nrhs, w.scal);
+    k_cocg_p<R><<<gv, BS, 0, s>>>((cx<R> *)w.q, (const cx<R> *)w.r, n, n, w.scal);
+}
+
+// COCG with reliable updates; b, e device FP64; returns iterations and updates
+template <typename R>
+static int run_cocg(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_solve_opts &o, double delta,
+                    long long *iters_out, long long *updates_out, double *worst_out, bool *conv_out) {
     Workspace &w = op->ws;
     hipStream_t s = op->ctx->stream;
     int64_t n = w.n;
@@ -806,12 +1126,159 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
     size_t pslot = (size_t)nrhs * NPART * 4;
     const DGrid<R> &g = op->G.view<R>();
     const OpData<R> &od = opd<R>(op);
-    k_init<R><<<gv, BS, 0, s>>>(g, od, (const c64 *)w.r64, (cx<R> *)w.r, (cx<R> *)w.q, (cx<R> *)w.d, (cx<R> *)w.z, n, n,
+    std::vector<double> bn(nrhs);
+    k_norm2<<<gv, BS, 0, s>>>((const c64 *)b, n, n, w.part + 2 * pslot, w.cnt + 2 * nrhs, w.bn);
+    MFD_HIP(hipMemcpyAsync(bn.data(), w.bn, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+    k_scal_reset<<<(unsigned)((nrhs + 63) / 64), 64, 0, s>>>(w.scal, nrhs);
+    MFD_HIP(hipMemsetAsync(w.z, 0, sizeof(cx<R>) * n * nrhs, s));
+    std::vector<int> mask(nrhs, 1), restart(nrhs, 1), done(nrhs, 0);
+    std::vector<double> relres(nrhs, 1.0);
+    std::vector<long long> last_ru(nrhs, 0);
+    int *mask_d = w.active, *aux_d = w.active + nrhs;   // mask and restart/done flags on the device
+    int *flush_d = w.active + 2 * nrhs;
+    auto up_mask = [&]() -> const int * {
+        hipMemcpyAsync(flush_d, mask.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s);
+        return flush_d;
+    };
+    auto residual = [&](double *worst) -> int {   // r64 = b − A e for masked RHS; marks converged ones
+        MFD_HIP(hipMemcpyAsync(mask_d, mask.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s));
+        launch_apply<double, MODE_A, EPI_RESID>(op, (const c64 *)e, (c64 *)w.r64, (const c64 *)b, nrhs,
+                                                w.part + 2 * pslot, w.cnt + 2 * nrhs, nullptr, w.rn, mask_d);
+        MFD_HIP(hipGetLastError());
+        MFD_HIP(hipMemcpyAsync(w.rn_h, w.rn, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+        MFD_HIP(hipStreamSynchronize(s));
+        *worst = 0;
+        for (int64_t r = 0; r < nrhs; ++r) {
+            if (mask[r]) {
+                double rel = bn[r] > 0 ? sqrt(w.rn_h[r] / bn[r]) : 0.0;
+                relres[r] = rel;
+                if (rel <= o.tol) done[r] = 1;
+            }
+            if (!done[r] && relres[r] > *worst) *worst = relres[r];
+        }
+        return MFD_OK;
+    };
+    std::vector<int> m2(nrhs);
+    auto replace = [&]() -> int {   // r (and p on restart) from r64 for masked, unconverged RHS
+        for (int64_t r = 0; r < nrhs; ++r) m2[r] = mask[r] && !done[r];
+        MFD_HIP(hipMemcpyAsync(mask_d, m2.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s));
+        MFD_HIP(hipMemcpyAsync(aux_d, restart.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s));
+        k_ru_replace<R><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (const c64 *)w.r64, (cx<R> *)w.r, (cx<R> *)w.q, n, n,
+                                          w.part + 2 * pslot, w.cnt + 2 * nrhs, w.scal, mask_d, aux_d, w.rn);
+        MFD_HIP(hipMemcpyAsync(aux_d, done.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s));
+        k_set_done<<<(unsigned)((nrhs + 63) / 64), 64, 0, s>>>(w.scal, aux_d, nrhs);
+        MFD_HIP(hipGetLastError());
+        return MFD_OK;
+    };
+    double worst = 0;
+    MFD_TRY(residual(&worst));
+    bool all = true;
+    for (int64_t r = 0; r < nrhs; ++r) all = all && done[r];
+    long long updates = 0, mx = 0;
+    if (!all) {
+        MFD_TRY(replace());
+        std::fill(restart.begin(), restart.end(), 0);
+        int chunk = o.check_every > 0 ? o.check_every : 32;
+        bool use_graph = o.use_graph != 0;
+        if (use_graph && (w.graph_iters != chunk || w.graph_kind != 2)) {
+            if (w.graph) { hipGraphExecDestroy(w.graph); w.graph = nullptr; }
+            hipGraph_t gr;
+            MFD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (int it = 0; it < chunk; ++it) launch_cocg_iter<R>(op, nrhs);
+            MFD_HIP(hipStreamEndCapture(s, &gr));
+            MFD_HIP(hipGraphInstantiate(&w.graph, gr, nullptr, nullptr, 0));
+            hipGraphDestroy(gr);
+            w.graph_iters = chunk;
+            w.graph_kind = 2;
+        }
+        while (true) {
+            if (use_graph) MFD_HIP(hipGraphLaunch(w.graph, s));
+            else
+                for (int it = 0; it < chunk; ++it) launch_cocg_iter<R>(op, nrhs);
+            MFD_HIP(hipGetLastError());
+            MFD_HIP(hipMemcpyAsync(w.scal_h, w.scal, sizeof(Scal) * nrhs, hipMemcpyDeviceToHost, s));
+            MFD_HIP(hipStreamSynchronize(s));
+            bool any = false;
+            mx = 0;
+            for (int64_t r = 0; r < nrhs; ++r) {
+                const Scal &S = w.scal_h[r];
+                if (S.it > mx) mx = S.it;
+                mask[r] = 0;
+                restart[r] = 0;
+                if (done[r]) continue;
+                double rn = sqrt(S.rnorm2);
+                double est = S.cn > 0 ? rn / S.cn * S.tau : 0.0;   // estimated true residual norm
+                // refresh when the recursive residual fell by δ below its running
+                // maximum, when convergence looks near, on breakdown, and as a
+                // guard when it has grown 1e4 past the last true residual or
+                // 2000 iterations passed without a refresh
+                bool trig = S.breakdown || rn < delta * S.tau_ref || est <= 0.5 * o.tol * sqrt(bn[r]) ||
+                            S.it >= o.maxit || (delta > 0 && (rn > 1e4 * S.cn || S.it - last_ru[r] >= 2000));
+                if (trig) { mask[r] = 1; restart[r] = S.breakdown || rn > 1e4 * S.cn; any = true; last_ru[r] = S.it; }
+            }
+            if (any) {
+                k_ru_flush<R><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (c64 *)e, (cx<R> *)w.z, n, n, up_mask());
+                MFD_TRY(residual(&worst));
+                MFD_TRY(replace());
+                ++updates;
+                if (getenv("MFD_SOLVE_TRACE"))
+                    fprintf(stderr, "[mfd cocg] it %lld update %lld worst residual %.3e\\n", mx, updates, worst);
+            }
+            all = true;
+            for (int64_t r = 0; r < nrhs; ++r) all = all && done[r];
+            if (all || mx >= o.maxit) break;
+        }
+        if (!all) {   // flush what the last iterations produced
+            std::fill(mask.begin(), mask.end(), 1);
+            k_ru_flush<R><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (c64 *)e, (cx<R> *)w.z, n, n, up_mask());
+            MFD_TRY(residual(&worst));
+        }
+    }
+    worst = 0;
+    for (int64_t r = 0; r < nrhs; ++r) worst = std::max(worst, relres[r]);
+    *iters_out = mx;
+    *updates_out = updates;
+    *worst_out = worst;
+    *conv_out = all;
+    return MFD_OK;
+}
+
+int op_set_tile_impl(mfd_op *op, int prec, int ty, int kz) {
+    if (!(ty == 0 || ty == 1 || ty == 2 || ty == 4) || kz < 1) { set_error("tile: TY in {0,1,2,4}, KZ >= 1"); return MFD_ERR_ARG; }
+    int *t = prec == MFD_FP64 ? op->tile64 : op->tile32;
+    t[0] = ty;
+    t[1] = kz;
+    op->ws.release();
+    return MFD_OK;
+}
+
+void solve_opts_default(mfd_solve_opts *o) {
+    o->tol = 1e-8;
+    o->maxit = 200000;
+    o->precision = MFD_FP64;
+    o->method = -1;
+    o->inner_tol = 1e-5;
+    o->check_every = 32;
+    o->use_graph = 1;
+}
+
+template <typename R>
+static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double rel_target, const int *active_d,
+                     long long budget, long long *iters_out, const void *b = nullptr, const void *e = nullptr,
+                     const double *bn = nullptr) {
+    Workspace &w = op->ws;
+    hipStream_t s = op->ctx->stream;
+    int64_t n = w.n;
+    dim3 gv(ngrid(n), (unsigned)nrhs);
+    size_t pslot = (size_t)nrhs * NPART * 4;
+    const DGrid<R> &g = op->G.view<R>();
+    const OpData<R> &od = opd<R>(op);
+    k_init<R><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (const c64 *)w.r64, (cx<R> *)w.r, (cx<R> *)w.q, (cx<R> *)w.d, (cx<R> *)w.z, n, n,
                                 w.part + 2 * pslot, w.cnt + 2 * nrhs, w.scal, rel_target, active_d, budget);
     MFD_HIP(hipGetLastError());
     int chunk = o->check_every > 0 ? o->check_every : 32;
     bool use_graph = o->use_graph != 0;
-    if (use_graph && w.graph_iters != chunk) {
+    if (use_graph && (w.graph_iters != chunk || w.graph_kind != 1)) {
         if (w.graph) { hipGraphExecDestroy(w.graph); w.graph = nullptr; }
         hipGraph_t gr;
         MFD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -820,8 +1287,13 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
         MFD_HIP(hipGraphInstantiate(&w.graph, gr, nullptr, nullptr, 0));
         hipGraphDestroy(gr);
         w.graph_iters = chunk;
+        w.graph_kind = 1;
     }
     long long done_it = 0;
+    const char *pe = getenv("MFD_PROBE_EVERY");
+    const long long probe = std::max<long long>(chunk, pe ? atoll(pe) : 512) / chunk * chunk;
+    double best_probe = 1e300;
+    int stall = 0;
     while (true) {
         if (use_graph) MFD_HIP(hipGraphLaunch(w.graph, s));
         else
@@ -838,6 +1310,24 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
         }
         *iters_out = mx;
         if (all || mx >= budget) break;
+        // FP32 cycles: probe the true FP64 residual every `probe` iterations and
+        // end the cycle once it stops improving (2x per probe window)
+        if (sizeof(R) == 4 && b && e && done_it % probe == 0) {
+            dim3 gv2(ngrid(n), (unsigned)nrhs);
+            k_probe_x<R><<<gv2, BS, 0, s>>>(op->G.d64, op->d64, (const c64 *)e, (const cx<R> *)w.z, (c64 *)w.et, n, n);
+            size_t ps = (size_t)nrhs * NPART * 4;
+            launch_apply<double, MODE_A, EPI_RESID>(op, (const c64 *)w.et, (c64 *)w.et2, (const c64 *)b, nrhs,
+                                                    w.part + 2 * ps, w.cnt + 2 * nrhs, nullptr, w.rn);
+            MFD_HIP(hipMemcpyAsync(w.rn_h, w.rn, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+            MFD_HIP(hipStreamSynchronize(s));
+            double worst = 0;
+            for (int64_t r = 0; r < nrhs; ++r) worst = std::max(worst, bn[r] > 0 ? sqrt(w.rn_h[r] / bn[r]) : 0.0);
+            if (getenv("MFD_SOLVE_TRACE")) fprintf(stderr, "[mfd probe] it %lld true residual %.3e\n", mx, worst);
+            if (worst > 0.5 * best_probe) {
+                if (++stall >= 2) break;
+            } else stall = 0;
+            best_probe = std::min(best_probe, worst);
+        }
     }
     return MFD_OK;
 }
@@ -855,6 +1345,31 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
     size_t pslot = (size_t)nrhs * NPART * 4;
     dim3 gv(ngrid(n), (unsigned)nrhs);
     hipEventRecord(op->ev0, s);
+    int method = o.method >= 0 ? o.method : (prec == MFD_MIXED ? MFD_COCG : MFD_SQMR);
+    if (method == MFD_COCG) {
+        long long it = 0, upd = 0;
+        double worst = 0;
+        bool conv = false;
+        // δ: FP32 recurrences are refreshed once the residual has fallen 100x;
+        // FP64 ones only when convergence is near
+        double delta = prec == MFD_MIXED ? 1e-2 : 0.0;
+        int rc = prec == MFD_MIXED ? run_cocg<float>(op, b, e, nrhs, o, delta, &it, &upd, &worst, &conv)
+                                   : run_cocg<double>(op, b, e, nrhs, o, delta, &it, &upd, &worst, &conv);
+        if (rc) return rc;
+        hipEventRecord(op->ev1, s);
+        hipEventSynchronize(op->ev1);
+        float ms = 0;
+        hipEventElapsedTime(&ms, op->ev0, op->ev1);
+        if (st) {
+            st->iterations = it;
+            st->outer_cycles = upd;
+            st->relres = worst;
+            st->seconds = ms * 1e-3;
+            st->converged = conv;
+        }
+        if (!conv) { set_error("solve did not reach the tolerance within maxit"); return MFD_ERR_NOCONV; }
+        return MFD_OK;
+    }
     k_norm2<<<gv, BS, 0, s>>>((const c64 *)b, n, n, w.part + 2 * pslot, w.cnt + 2 * nrhs, w.bn);
     std::vector<double> bn(nrhs);
     MFD_HIP(hipMemcpyAsync(bn.data(), w.bn, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
@@ -880,6 +1395,11 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
         if (all) { conv = true; break; }
         if (total_it >= o.maxit) break;
         MFD_HIP(hipMemcpyAsync(active_d, active.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s));
+        if (op->gd) {
+            // the inner system is A' z = r' with r' = r + (1/ω²) ε G W Div r
+            k_gd_psi<<<gv, BS, 0, s>>>(op->G.d64, op->d64.wnode, (const c64 *)w.r64, (c64 *)w.psi, n);
+            k_gd_rhs<<<gv, BS, 0, s>>>(op->G.d64, op->d64, (const c64 *)w.psi, (c64 *)w.r64, n, cinv(op->d64.om2));
+        }
         // inner target, relative to this cycle's residual: enough to finish
         // (0.1 tol / worst), never looser than 1e-2 so every restart makes
         // progress, and in mixed precision never tighter than inner_tol
@@ -889,7 +1409,8 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
         if (prec == MFD_MIXED) target = std::max(target, o.inner_tol);
         long long it = 0;
         int rc;
-        if (prec == MFD_MIXED) rc = run_inner<float>(op, nrhs, &o, target, active_d, o.maxit - total_it, &it);
+        if (prec == MFD_MIXED)
+            rc = run_inner<float>(op, nrhs, &o, target, active_d, o.maxit - total_it, &it, b, e, bn.data());
         else rc = run_inner<double>(op, nrhs, &o, target, active_d, o.maxit - total_it, &it);
         if (rc) return rc;
         total_it += it;
@@ -901,7 +1422,7 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
                     cycles, prec == MFD_MIXED ? "fp32" : "fp64", worst, target, it,
                     w.scal_h[0].cn > 0 ? w.scal_h[0].tau / w.scal_h[0].cn : 0.0, w.scal_h[0].stag);
         }
-        if (prec == MFD_MIXED) k_upd_e<float><<<gv, BS, 0, s>>>(op->G.d32, op->d32, (c64 *)e, (const c32 *)w.z, n, n);
+        if (prec == MFD_MIXED) k_upd_e<float><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (c64 *)e, (const c32 *)w.z, n, n);
         else k_upd_e<double><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (c64 *)e, (const c64 *)w.z, n, n);
         MFD_HIP(hipGetLastError());
         if (cycles > 200) break;

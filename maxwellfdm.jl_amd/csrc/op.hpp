@@ -23,7 +23,8 @@ struct OpData {
     const cx<R> *epsd;    // [M][3] diagonal ε at E_c locations
     const cx<R> *epso;    // [M][3] (xy, xz, yz) at primal corners, or null
     const cx<R> *muinv;   // [M][3] 1/μ_cc at H_c locations, or null
-    const cx<R> *q;       // [M][3] Jacobi scaling diag(A)^-1/2
+    const cx<R> *q;       // [M][3] Jacobi scaling diag(A)^-1/2 (of A' when the grad-div form is on)
+    const cx<R> *wnode;   // [M] grad-div weight W = 1/ε̄² at the primal nodes
     cx<R> om2;            // ω²
 };
 
@@ -35,6 +36,8 @@ struct Workspace {
     // inner vectors (precision of the inner solve), nrhs each
     void *r = nullptr, *q = nullptr, *t = nullptr, *d = nullptr, *z = nullptr;
     void *r64 = nullptr;      // outer FP64 residual
+    void *psi = nullptr;      // FP64 node scratch for the grad-div residual transform
+    void *et = nullptr, *et2 = nullptr;   // FP64 scratch for true-residual probes
     double *part = nullptr;   // reduction partials [slot][rhs][NPART][4]
     unsigned *cnt = nullptr;  // [slot][rhs]
     Scal *scal = nullptr;     // [rhs]
@@ -42,9 +45,10 @@ struct Workspace {
     double *bn = nullptr;     // [rhs] ||b||²
     double *rn = nullptr;     // [rhs] ||r||²
     double *rn_h = nullptr;   // pinned
-    int *active = nullptr;    // [rhs] RHS still above the tolerance
+    int *active = nullptr;    // [3 rhs] per-RHS masks and flags
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0;
+    int graph_kind = 0;       // 1 = SQMR iteration, 2 = COCG iteration
     void release();
 };
 
@@ -64,4 +68,6 @@ struct mfd_op {
     // selects the direct kernel
     int tile64[2] = {4, 16};
     int tile32[2] = {4, 16};
+    int gd = 0;               // grad-div augmented inner operator in use
+    int gd_ok = 0;            // eligible (diagonal ε, μ = 1)
 };

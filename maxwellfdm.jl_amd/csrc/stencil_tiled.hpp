@@ -1,45 +1,125 @@
 // Tiled 2.5D curl-curl stencil for gfx950 (diagonal ε, μ = 1).
 //
+// Vector storage precision R (complex FP64 or FP32 iterates in HBM); the
+// operator data (ε, Jacobi scaling, grad-div weights), the 1D metric tables,
+// the LDS staging and all arithmetic are FP64, so the FP32 solver iterates
+// against exactly the operator the FP64 residual uses — only the vectors are
+// rounded, once, at the store.  The kernel is HBM-bound: FP64 math is free.
+//
 // A block owns a 64 × TY column tile in x-y (one 64-lane wavefront per x row,
-// so global loads of a row are contiguous) and marches over a chunk of KZ
-// z-planes.  Per plane it stages the scaled iterate X = Rf ∘ x of plane k+1
-// (tile + one-cell halo ring) into LDS, computes H = Cₑ X of plane k on the
-// tile + (−1) halo into LDS, and forms y at plane k from H(k), its −x/−y
-// neighbours (LDS) and H(k−1) (registers of the owning thread).  Each iterate
-// value is read from HBM once per block plus the halo ring; H is never
-// written to memory.  Included by op.hip (needs its helpers).
+// so a row's loads are contiguous) and marches over a chunk of KZ z-planes.
+// Per plane it stages the scaled iterate X = Rf ∘ x of plane k+1 (tile + a
+// one-cell halo ring) in LDS, computes H = Cₑ X of plane k on the tile plus
+// its −x/−y halo in LDS, and forms y at plane k from H(k), its −x/−y
+// neighbours and H(k−1) (registers of the owning thread).  Every 1D metric
+// factor a thread needs (x from its column, y from its row) is loaded once
+// into registers; the z factors of a plane are block-uniform.  H is never
+// written to memory.
+//
+// GD adds the grad-div term of the augmented inner operator
+//     A' = Cₘ Cₑ − ω² ε − ε G W Div ε,
+// staging Y = ε ∘ X next to X and keeping ψ = W ∘ Div(Y) for planes k and k+1
+// in LDS (G: forward differences to E edges, Div: backward differences to the
+// primal nodes, the same metric as the curls).  Included by op.hip.
 
 template <int TY>
 struct TileCfg {
     static constexpr int TX = 64;
-    static constexpr int NT = TX * TY;        // threads per block
-    static constexpr int XW = TX + 2;         // staged row width (halo both sides)
+    static constexpr int NT = TX * TY;   // threads per block
+    static constexpr int XW = TX + 2;    // staged row width (halo both sides)
     static constexpr int XH = TY + 2;
-    static constexpr int HW = TX + 1;         // H rows: −1 halo on the low side
-    static constexpr int HH_ = TY + 1;
+    static constexpr int HW = TX + 1;    // H rows: −1 halo on the low side
+    static constexpr int HHt = TY + 1;
+    static constexpr int PW = TX + 1;    // ψ rows: +1 halo on the high side
+    static constexpr int PHt = TY + 1;
 };
 
-// a staged X value at global (gi, gj, gk): ghosts via ldX, points beyond the
-// one-cell ring are never used and read as zero
-template <typename R, int MODE>
-__device__ inline cx<R> stage_ld(const DGrid<R> &g, const OpData<R> &od, const cx<R> *__restrict__ x, int c, int gi,
-                                 int gj, int gk) {
-    if (gi < -1 || gi > g.N[0] || gj < -1 || gj > g.N[1] || gk < -1 || gk > g.N[2]) return czero<R>();
-    return ldX<R, MODE>(g, od, x, c, gi, gj, gk);
+// wrap a tile coordinate along axis a: returns false when the point is a
+// non-Bloch ghost (value zero) or lies beyond the one-cell ring
+template <typename R>
+__device__ inline bool wrap_ax(const DGrid<R> &g, int a, int v, int &w, cx<R> &ph) {
+    int N = g.N[a];
+    if (v < -1 || v > N) return false;
+    if (v == -1) {
+        if (!g.bloch[a]) return false;
+        w = N - 1;
+        ph = ph * g.phinv[a];
+    } else if (v == N) {
+        if (!g.bloch[a]) return false;
+        w = 0;
+        ph = ph * g.ph[a];
+    } else w = v;
+    return true;
 }
 
-template <typename R, int MODE, int EPI, int TY>
-__global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<R> g, OpData<R> od, const cx<R> *__restrict__ xall,
+// 1D term of the curl-curl diagonal along axis a at index ia (μ = 1)
+template <typename R>
+__device__ inline cx<R> axis_term(const DGrid<R> &g, int a, int ia) {
+    int Na = g.N[a];
+    cx<R> ie = g.sdlinv[PRIM][a][ia];
+    if (Na == 1 && g.bloch[a]) {
+        cx<R> f = mk<R>((R)2, (R)0) - g.ph[a] - g.phinv[a];
+        return (ie * g.sdlinv[DUAL][a][0]) * f;
+    }
+    if (ia == 0 && !g.bloch[a]) return ie * g.sdlinv[DUAL][a][0];
+    int im = ia > 0 ? ia - 1 : Na - 1;
+    return ie * (g.sdlinv[DUAL][a][ia] + g.sdlinv[DUAL][a][im]);
+}
+
+// per-thread cached description of one staged point (fixed x-y position)
+template <typename R>
+struct StagePt {
+    int gi, gj;        // wrapped indices
+    bool ok;           // not a zero ghost / out of ring
+    cx<R> ph;          // x-y wrap phase
+    cx<R> tinv[3];     // x·y part of T⁻¹ per component
+};
+
+template <typename R>
+__device__ inline void stage_pt_init(const DGrid<R> &g, int vi, int vj, StagePt<R> &s) {
+    s.ph = cone<R>();
+    s.ok = wrap_ax(g, 0, vi, s.gi, s.ph) && wrap_ax(g, 1, vj, s.gj, s.ph);
+    if (!s.ok) { s.gi = 0; s.gj = 0; }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        s.tinv[c] = g.ssqinv[c == 0 ? DUAL : PRIM][0][s.gi] * g.ssqinv[c == 1 ? DUAL : PRIM][1][s.gj];
+}
+
+// per-plane (block-uniform) z data
+template <typename R>
+struct PlaneZ {
+    int gk;
+    bool ok;
+    cx<R> ph;
+    cx<R> tinv[3];
+};
+
+template <typename R>
+__device__ inline void plane_init(const DGrid<R> &g, int vk, PlaneZ<R> &z) {
+    z.ph = cone<R>();
+    z.ok = wrap_ax(g, 2, vk, z.gk, z.ph);
+    if (!z.ok) z.gk = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) z.tinv[c] = g.ssqinv[c == 2 ? DUAL : PRIM][2][z.gk];
+}
+
+template <typename R, int MODE, int EPI, int TY, bool GD>
+__global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData<double> od, c64 om2,
+                                                         const cx<R> *__restrict__ xall,
                                                          cx<R> *__restrict__ yall, const c64 *__restrict__ ball,
                                                          int64_t vs, int KZ, int nch, double *part, unsigned *cnt,
-                                                         Scal *scal, double *rn) {
+                                                         Scal *scal, double *rn, const int *__restrict__ mask = nullptr) {
     using C = TileCfg<TY>;
-    __shared__ cx<R> Xs[2][3][C::XH][C::XW];
-    __shared__ cx<R> Hs[3][C::HH_][C::HW];
+    constexpr int GB = GD ? 2 : 1, GH = GD ? C::XH : 1, GW = GD ? C::XW : 1;
+    __shared__ c64 Xs[2][3][C::XH][C::XW];
+    __shared__ c64 Ys[GB][3][GH][GW];
+    __shared__ c64 Hs[3][C::HHt][C::HW];
+    __shared__ c64 Ps[GB][GD ? C::PHt : 1][GD ? C::PW : 1];
     const int rhs = blockIdx.z / nch, chunk = blockIdx.z - rhs * nch;
     const cx<R> *x = xall + rhs * vs;
     cx<R> *y = yall + rhs * vs;
     bool skip = false;
+    if (EPI == EPI_RESID && mask && !mask[rhs]) return;
     if (EPI == EPI_DOT) {
         if (scal[rhs].done) return;
         skip = scal[rhs].conv != 0;
@@ -49,124 +129,233 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<R> g, OpData<R> o
     const int i = i0 + tx, j = j0 + ty;
     const bool own = i < g.N[0] && j < g.N[1];
     const int k0 = chunk * KZ, k1 = min(k0 + KZ, g.N[2]);
-    const int nring = 2 * C::XW + 2 * TY;
+    const int64_t NxNy = (int64_t)g.N[0] * g.N[1];
     double acc[2] = {0.0, 0.0};
 
-    // stage plane gk into buffer b: interior by its owner (raw x, q kept), ring by the first threads
-    auto stage = [&](int b, int gk, cx<R> (&xr)[3], cx<R> (&qr)[3]) {
+    // staged points of this thread: its interior point (slot 0) and its share
+    // of the halo ring (slots 1..NRS; the ring has 2·XW + 2·TY points)
+    constexpr int nring = 2 * C::XW + 2 * TY;
+    constexpr int NRS = (nring + C::NT - 1) / C::NT;
+    StagePt<double> sp[1 + NRS];
+    int sjj[1 + NRS], sii[1 + NRS];
+    bool sact[1 + NRS];
+    sii[0] = tx + 1;
+    sjj[0] = ty + 1;
+    sact[0] = true;
+    stage_pt_init(g, i, j, sp[0]);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            cx<R> v = czero<R>();
-            if (own && gk >= 0 && gk < g.N[2]) {
-                int64_t p = i + (int64_t)g.N[0] * (j + (int64_t)g.N[1] * gk);
-                cx<R> xv = x[dofidx(g, c, p)];
-                xr[c] = xv;
-                if (MODE == MODE_P) {
-                    cx<R> qv = od.q[3 * p + c];
-                    qr[c] = qv;
-                    v = xv * (qv * tfac(g, c, i, j, gk, true));
-                } else v = xv;
-            } else if (own) {
-                v = stage_ld<R, MODE>(g, od, x, c, i, j, gk);
-            }
-            Xs[b][c][ty + 1][tx + 1] = v;
-        }
-        for (int t = tid; t < nring; t += C::NT) {
-            int ii, jj;
-            if (t < C::XW) { ii = t; jj = 0; }
-            else if (t < 2 * C::XW) { ii = t - C::XW; jj = C::XH - 1; }
-            else { int u = t - 2 * C::XW; ii = (u & 1) ? C::XW - 1 : 0; jj = 1 + (u >> 1); }
+    for (int q = 0; q < NRS; ++q) {
+        int t = tid + q * C::NT, ii, jj;
+        sact[1 + q] = t < nring;
+        if (!sact[1 + q]) t = 0;
+        if (t < C::XW) { ii = t; jj = 0; }
+        else if (t < 2 * C::XW) { ii = t - C::XW; jj = C::XH - 1; }
+        else { int u = t - 2 * C::XW; ii = (u & 1) ? C::XW - 1 : 0; jj = 1 + (u >> 1); }
+        sii[1 + q] = ii;
+        sjj[1 + q] = jj;
+        stage_pt_init(g, i0 - 1 + ii, j0 - 1 + jj, sp[1 + q]);
+    }
+    // owner's output factors (x from the column, y from the row)
+    const int io = own ? i : 0, jo = own ? j : 0;
+    const c64 exo = g.sdlinv[PRIM][0][io], eyo = g.sdlinv[PRIM][1][jo];
+    c64 Txy[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) Xs[b][c][jj][ii] = stage_ld<R, MODE>(g, od, x, c, i0 - 1 + ii, j0 - 1 + jj, gk);
-        }
-        // interior points of the tile that fall outside the grid
-        if (!own) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) Xs[b][c][ty + 1][tx + 1] = stage_ld<R, MODE>(g, od, x, c, i, j, gk);
-        }
-    };
+    for (int c = 0; c < 3; ++c)
+        Txy[c] = g.ssq[c == 0 ? DUAL : PRIM][0][io] * g.ssq[c == 1 ? DUAL : PRIM][1][jo];
+    const c64 tmx = axis_term(g, 0, io), tmy = axis_term(g, 1, jo);
+    const c64 gdx = g.sdlinv[DUAL][0][io], gdy = g.sdlinv[DUAL][1][jo];
 
-    // H_c at LDS position (hi, hj) of plane gk from buffers bk (plane gk) and bk1 (gk+1)
-    auto hval = [&](int bk, int bk1, int hi, int hj, int gk, cx<R> (&h)[3]) {
+    // H points of this thread (fixed across planes): LDS (hi, hj) -> global (i0-1+hi, j0-1+hj)
+    constexpr int NHP = (C::HHt * C::HW + C::NT - 1) / C::NT;
+    c64 hax[NHP], hay[NHP];
+    bool hok[NHP];
+#pragma unroll
+    for (int r = 0; r < NHP; ++r) {
+        int t = tid + r * C::NT;
+        int hj = t / C::HW, hi = t - hj * C::HW;
         int ri = i0 - 1 + hi, rj = j0 - 1 + hj;
-        if (ri >= g.N[0] || rj >= g.N[1] || (ri < 0 && !g.bloch[0]) || (rj < 0 && !g.bloch[1]) ||
-            (gk < 0 && !g.bloch[2])) {
-            h[0] = h[1] = h[2] = czero<R>();
-            return;
-        }
-        int ix = ri < 0 ? g.N[0] - 1 : ri, iy = rj < 0 ? g.N[1] - 1 : rj, iz = gk < 0 ? g.N[2] - 1 : gk;
-        cx<R> ax = g.sdlinv[DUAL][0][ix], ay = g.sdlinv[DUAL][1][iy], az = g.sdlinv[DUAL][2][iz];
-        cx<R> X0[3], Xpx[3], Xpy[3], Xpz[3];
+        hok[r] = t < C::HHt * C::HW && ri < g.N[0] && rj < g.N[1] && (ri >= 0 || g.bloch[0]) && (rj >= 0 || g.bloch[1]);
+        int ix = ri < 0 ? g.N[0] - 1 : (ri < g.N[0] ? ri : 0), iy = rj < 0 ? g.N[1] - 1 : (rj < g.N[1] ? rj : 0);
+        hax[r] = g.sdlinv[DUAL][0][ix];
+        hay[r] = g.sdlinv[DUAL][1][iy];
+    }
+    // ψ nodes of this thread: LDS (pi, pj) -> global (i0+pi, j0+pj), pi in [0,64], pj in [0,TY]
+    constexpr int NPP = GD ? (C::PHt * C::PW + C::NT - 1) / C::NT : 1;
+    c64 pex[NPP], pey[NPP], pph[NPP];
+    int pidx[NPP];
+    bool pok[NPP];
+    if (GD) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            X0[c] = Xs[bk][c][hj][hi];
-            Xpx[c] = Xs[bk][c][hj][hi + 1];
-            Xpy[c] = Xs[bk][c][hj + 1][hi];
-            Xpz[c] = Xs[bk1][c][hj][hi];
+        for (int r = 0; r < NPP; ++r) {
+            int t = tid + r * C::NT;
+            int pj = t / C::PW, pi = t - pj * C::PW;
+            int gi = 0, gj = 0;
+            c64 ph = cone<double>();
+            pok[r] = t < C::PHt * C::PW && wrap_ax(g, 0, i0 + pi, gi, ph) && wrap_ax(g, 1, j0 + pj, gj, ph);
+            if (!pok[r]) { gi = 0; gj = 0; }
+            pidx[r] = gi + g.N[0] * gj;
+            pph[r] = ph;
+            pex[r] = g.sdlinv[PRIM][0][gi];
+            pey[r] = g.sdlinv[PRIM][1][gj];
         }
-        h[0] = ay * (Xpy[2] - X0[2]) - az * (Xpz[1] - X0[1]);
-        h[1] = az * (Xpz[0] - X0[0]) - ax * (Xpx[2] - X0[2]);
-        h[2] = ax * (Xpx[1] - X0[1]) - ay * (Xpy[0] - X0[0]);
+    }
+
+    // stage plane vk into buffer b; the owner keeps raw x, q and ε of its point
+    auto stage = [&](int b, int vk, c64 (&xr)[3], c64 (&qr)[3], c64 (&er)[3]) {
+        PlaneZ<double> z;
+        plane_init(g, vk, z);
+#pragma unroll
+        for (int s = 0; s < 1 + NRS; ++s) {
+            if (!sact[s]) continue;
+            const StagePt<double> &P = sp[s];
+            bool ok = P.ok && z.ok;
+            int64_t p = P.gi + (int64_t)g.N[0] * P.gj + NxNy * z.gk;
+            c64 ph = P.ph * z.ph;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                c64 v = czero<double>(), yv = czero<double>();
+                if (ok) {
+                    c64 xv = cvt<double>(x[dofidx(g, c, p)]);
+                    c64 qv = cone<double>();
+                    if (MODE == MODE_P) {
+                        qv = cvt<double>(od.q[3 * p + c]);
+                        v = (xv * qv) * (P.tinv[c] * z.tinv[c]);
+                    } else v = xv;
+                    c64 ev = cone<double>();
+                    if (GD || (MODE == MODE_A && s == 0)) ev = cvt<double>(od.epsd[3 * p + c]);
+                    if (GD) yv = ev * v;
+                    if (s == 0) { xr[c] = xv; qr[c] = qv; er[c] = ev; }
+                    v = v * ph;
+                    yv = yv * ph;
+                }
+                Xs[b][c][sjj[s]][sii[s]] = v;
+                if (GD) Ys[b][c][sjj[s]][sii[s]] = yv;
+            }
+        }
     };
 
-    cx<R> xc[3], qc[3], xn[3], qn[3];   // raw x and q of the owner at planes k and k+1
-    cx<R> hpx = czero<R>(), hpy = czero<R>();
+    // H of plane vk (buffers bk: plane vk, bn: plane vk+1) for all H points
+    auto hplane = [&](int bk, int bn, int vk) {
+        PlaneZ<double> z;
+        plane_init(g, vk, z);
+        const c64 az = g.sdlinv[DUAL][2][z.gk];
+        const bool zok = vk >= 0 || g.bloch[2];
+#pragma unroll
+        for (int r = 0; r < NHP; ++r) {
+            int t = tid + r * C::NT;
+            if (t >= C::HHt * C::HW) break;
+            int hj = t / C::HW, hi = t - hj * C::HW;
+            c64 h0 = czero<double>(), h1 = czero<double>(), h2 = czero<double>();
+            if (hok[r] && zok) {
+                c64 ax = hax[r], ay = hay[r];
+                c64 X0x = Xs[bk][0][hj][hi], X0y = Xs[bk][1][hj][hi], X0z = Xs[bk][2][hj][hi];
+                h0 = ay * (Xs[bk][2][hj + 1][hi] - X0z) - az * (Xs[bn][1][hj][hi] - X0y);
+                h1 = az * (Xs[bn][0][hj][hi] - X0x) - ax * (Xs[bk][2][hj][hi + 1] - X0z);
+                h2 = ax * (Xs[bk][1][hj][hi + 1] - X0y) - ay * (Xs[bk][0][hj + 1][hi] - X0x);
+            }
+            Hs[0][hj][hi] = h0;
+            Hs[1][hj][hi] = h1;
+            Hs[2][hj][hi] = h2;
+        }
+    };
+
+    // ψ = W Div(Y) of node plane vk (Y buffers bk: plane vk, bm: plane vk−1) into Ps[pb]
+    auto pplane = [&](int bk, int bm, int vk, int pb) {
+        int gk = 0;
+        c64 phz = cone<double>();
+        bool zok = wrap_ax(g, 2, vk, gk, phz);
+        const c64 ez = g.sdlinv[PRIM][2][gk];
+#pragma unroll
+        for (int r = 0; r < NPP; ++r) {
+            int t = tid + r * C::NT;
+            if (t >= C::PHt * C::PW) break;
+            int pj = t / C::PW, pi = t - pj * C::PW;
+            c64 psi = czero<double>();
+            if (pok[r] && zok) {
+                int ii = pi + 1, jj = pj + 1;   // node (i0+pi, j0+pj) in staged coordinates
+                c64 dv = pex[r] * (Ys[bk][0][jj][ii] - Ys[bk][0][jj][ii - 1]) +
+                           pey[r] * (Ys[bk][1][jj][ii] - Ys[bk][1][jj - 1][ii]) +
+                           ez * (Ys[bk][2][jj][ii] - Ys[bm][2][jj][ii]);
+                // the staged Y already carries the wrap phase of a ghost node
+                psi = cvt<double>(od.wnode[pidx[r] + (int64_t)g.N[0] * g.N[1] * gk]) * dv;
+            }
+            Ps[pb][pj][pi] = psi;
+        }
+    };
+
+    c64 xc[3], qc[3], ec[3], xn[3], qn[3], en[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { xc[c] = xn[c] = czero<double>(); qc[c] = qn[c] = ec[c] = en[c] = cone<double>(); }
+    c64 hpx = czero<double>(), hpy = czero<double>();
     if (!skip && k0 < k1) {
-        // prologue: planes k0−1 and k0, H_x/H_y of plane k0−1 at the owned point
-        stage(0, k0 - 1, xn, qn);
-        stage(1, k0, xc, qc);
+        stage(0, k0 - 1, xn, qn, en);
+        stage(1, k0, xc, qc, ec);
         __syncthreads();
-        cx<R> h[3];
-        hval(0, 1, tx + 1, ty + 1, k0 - 1, h);
-        hpx = h[0];
-        hpy = h[1];
+        // H_x, H_y of plane k0−1 at the owned point (zero across a non-Bloch z ghost)
+        if (own && (k0 > 0 || g.bloch[2])) {
+            int gkm = k0 > 0 ? k0 - 1 : g.N[2] - 1;
+            c64 az = g.sdlinv[DUAL][2][gkm];
+            int hi = tx + 1, hj = ty + 1;
+            c64 X0x = Xs[0][0][hj][hi], X0y = Xs[0][1][hj][hi], X0z = Xs[0][2][hj][hi];
+            c64 ax = g.sdlinv[DUAL][0][i], ay = g.sdlinv[DUAL][1][j];
+            hpx = ay * (Xs[0][2][hj + 1][hi] - X0z) - az * (Xs[1][1][hj][hi] - X0y);
+            hpy = az * (Xs[1][0][hj][hi] - X0x) - ax * (Xs[0][2][hj][hi + 1] - X0z);
+        }
+        if (GD) pplane(1, 0, k0, k0 & 1);
         for (int k = k0; k < k1; ++k) {
             const int bk = (k - k0 + 1) & 1, bn = bk ^ 1;
-            __syncthreads();                       // everyone is done with buffer bn and Hs
-            stage(bn, k + 1, xn, qn);
+            __syncthreads();   // buffer bn, Hs and Ps[(k+1)&1] are free
+            stage(bn, k + 1, xn, qn, en);
             __syncthreads();
-            for (int t = tid; t < C::HH_ * C::HW; t += C::NT) {
-                int hj = t / C::HW, hi = t - hj * C::HW;
-                cx<R> hh[3];
-                hval(bk, bn, hi, hj, k, hh);
-#pragma unroll
-                for (int c = 0; c < 3; ++c) Hs[c][hj][hi] = hh[c];
-            }
+            hplane(bk, bn, k);
+            if (GD) pplane(bn, bk, k + 1, (k + 1) & 1);
             __syncthreads();
             if (own) {
-                cx<R> H0[3], Hxm[3], Hym[3];
+                c64 H0[3], Hxm[3], Hym[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     H0[c] = Hs[c][ty + 1][tx + 1];
                     Hxm[c] = Hs[c][ty + 1][tx];
                     Hym[c] = Hs[c][ty][tx + 1];
                 }
-                cx<R> ex = g.sdlinv[PRIM][0][i], ey = g.sdlinv[PRIM][1][j], ez = g.sdlinv[PRIM][2][k];
-                cx<R> cc[3];
-                cc[0] = ey * (H0[2] - Hym[2]) - ez * (H0[1] - hpy);
-                cc[1] = ez * (H0[0] - hpx) - ex * (H0[2] - Hxm[2]);
-                cc[2] = ex * (H0[1] - Hxm[1]) - ey * (H0[0] - Hym[0]);
+                const c64 ez = g.sdlinv[PRIM][2][k];
+                c64 cc[3];
+                cc[0] = eyo * (H0[2] - Hym[2]) - ez * (H0[1] - hpy);
+                cc[1] = ez * (H0[0] - hpx) - exo * (H0[2] - Hxm[2]);
+                cc[2] = exo * (H0[1] - Hxm[1]) - eyo * (H0[0] - Hym[0]);
                 hpx = H0[0];
                 hpy = H0[1];
-                int64_t p = i + (int64_t)g.N[0] * (j + (int64_t)g.N[1] * k);
+                if (GD) {   // cc −= ε G ψ (forward differences of ψ to the E edges)
+                    const int pc = k & 1, pn = (k + 1) & 1;
+                    c64 p0 = Ps[pc][ty][tx];
+                    cc[0] = cc[0] - ec[0] * (gdx * (Ps[pc][ty][tx + 1] - p0));
+                    cc[1] = cc[1] - ec[1] * (gdy * (Ps[pc][ty + 1][tx] - p0));
+                    cc[2] = cc[2] - ec[2] * (g.sdlinv[DUAL][2][k] * (Ps[pn][ty][tx] - p0));
+                }
+                int64_t p = i + (int64_t)g.N[0] * j + NxNy * k;
+                c64 tzm = axis_term(g, 2, k);
 #pragma unroll
                 for (int u = 0; u < 3; ++u) {
-                    cx<R> out;
+                    c64 out;
                     if (MODE == MODE_A) {
-                        out = cc[u] - od.om2 * (od.epsd[3 * p + u] * xc[u]);
+                        out = cc[u] - om2 * (ec[u] * xc[u]);
                     } else {
-                        cx<R> lf = qc[u] * tfac(g, u, i, j, k, false);
-                        cx<R> cd = ccdiag<R, false>(g, od, u, i, j, k);
-                        out = lf * cc[u] + (cone<R>() - cd * (qc[u] * qc[u])) * xc[u];
+                        c64 lf = qc[u] * (Txy[u] * g.ssq[u == 2 ? DUAL : PRIM][2][k]);
+                        if (GD) {
+                            out = lf * cc[u] - om2 * ((ec[u] * (qc[u] * qc[u])) * xc[u]);
+                        } else {
+                            c64 cd = u == 0 ? tmy + tzm : (u == 1 ? tzm + tmx : tmx + tmy);
+                            out = lf * cc[u] + (cone<double>() - cd * (qc[u] * qc[u])) * xc[u];
+                        }
                     }
                     int64_t e = dofidx(g, u, p);
                     if (EPI == EPI_RESID) {
                         c64 bb = ball[rhs * vs + e];
-                        cx<R> rr = mk<R>((R)bb.re - out.re, (R)bb.im - out.im);
-                        y[e] = rr;
+                        c64 rr = mk<double>(bb.re - out.re, bb.im - out.im);
+                        y[e] = cvt<R>(rr);
                         acc[0] += (double)rr.re * rr.re + (double)rr.im * rr.im;
                     } else {
-                        y[e] = out;
+                        y[e] = cvt<R>(out);
                         if (EPI == EPI_DOT) {
                             acc[0] += (double)xc[u].re * out.re - (double)xc[u].im * out.im;
                             acc[1] += (double)xc[u].re * out.im + (double)xc[u].im * out.re;
@@ -175,7 +364,7 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<R> g, OpData<R> o
                 }
             }
 #pragma unroll
-            for (int c = 0; c < 3; ++c) { xc[c] = xn[c]; qc[c] = qn[c]; }
+            for (int c = 0; c < 3; ++c) { xc[c] = xn[c]; qc[c] = qn[c]; ec[c] = en[c]; }
         }
     }
     if (EPI == EPI_STORE) return;

@@ -14,6 +14,7 @@ LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "lib", "libmfdfd.so"))
 
 MFD_OK, MFD_ERR_ARG, MFD_ERR_UNSUPPORTED, MFD_ERR_HIP, MFD_ERR_NOCONV, MFD_ERR_ALLOC = range(6)
 MFD_FP64, MFD_MIXED = 0, 1
+MFD_SQMR, MFD_COCG = 0, 1
 
 
 class c64(C.Structure):
@@ -33,7 +34,7 @@ class mfd_shape(C.Structure):
 
 
 class mfd_solve_opts(C.Structure):
-    _fields_ = [("tol", C.c_double), ("maxit", C.c_int64), ("precision", C.c_int32), ("pad", C.c_int32),
+    _fields_ = [("tol", C.c_double), ("maxit", C.c_int64), ("precision", C.c_int32), ("method", C.c_int32),
                 ("inner_tol", C.c_double), ("check_every", C.c_int32), ("use_graph", C.c_int32)]
 
 
@@ -87,6 +88,10 @@ SIGNATURES = {
     "mfd_op_destroy": (C.c_int, [_P]),
     "mfd_op_apply": (C.c_int, [_P, _P, _P, C.c_int64]),
     "mfd_op_apply_dev": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int]),
+    "mfd_op_set_graddiv": (C.c_int, [_P, C.c_int]),
+    "mfd_op_set_tile": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
+    "mfd_op_autotune": (C.c_int, [_P, C.c_int64, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_double)]),
     "mfd_op_bench_apply": (C.c_int, [_P, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "mfd_solve_opts_default": (None, [C.POINTER(mfd_solve_opts)]),
     "mfd_solve": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(mfd_solve_opts), C.POINTER(mfd_solve_stats)]),

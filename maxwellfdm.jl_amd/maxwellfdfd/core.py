@@ -430,6 +430,22 @@ class MaxwellOperator:
         check(lib().mfd_op_apply_dev(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), nrhs,
                                      L.MFD_FP64 if precision == "fp64" else L.MFD_MIXED))
 
+    def set_graddiv(self, enable=True):
+        """Grad-div augmented inner operator of the solver (see mfd_op_set_graddiv)."""
+        check(lib().mfd_op_set_graddiv(self._h, int(bool(enable))))
+
+    def set_tile(self, precision, ty, kz):
+        """Stencil tile shape: TY rows per block (1, 2, 4; 0 = direct kernel), KZ planes per block."""
+        check(lib().mfd_op_set_tile(self._h, L.MFD_FP64 if precision == "fp64" else L.MFD_MIXED, ty, kz))
+
+    def autotune(self, nrhs=1, iters=10):
+        """Pick the stencil tile shape by timing; returns ((TY, KZ) fp64, (TY, KZ) fp32, (ms64, ms32))."""
+        t64 = (C.c_int * 2)()
+        t32 = (C.c_int * 2)()
+        ms = (C.c_double * 2)()
+        check(lib().mfd_op_autotune(self._h, nrhs, iters, t64, t32, ms))
+        return tuple(t64), tuple(t32), tuple(ms)
+
     def bench_apply(self, nrhs=1, precision="fp64", iters=20):
         ms = C.c_double()
         check(lib().mfd_op_bench_apply(self._h, nrhs, L.MFD_FP64 if precision == "fp64" else L.MFD_MIXED, iters,
@@ -437,9 +453,11 @@ class MaxwellOperator:
         return ms.value
 
     @staticmethod
-    def _opts(tol, precision, maxit, inner_tol, check_every, use_graph):
+    def _opts(tol, precision, maxit, inner_tol, check_every, use_graph, method=None):
         o = L.mfd_solve_opts()
         lib().mfd_solve_opts_default(C.byref(o))
+        if method is not None:
+            o.method = {"sqmr": L.MFD_SQMR, "cocg": L.MFD_COCG}[method]
         o.tol = tol
         o.precision = L.MFD_MIXED if precision == "mixed" else L.MFD_FP64
         if maxit is not None:
@@ -452,11 +470,11 @@ class MaxwellOperator:
         return o
 
     def solve(self, b, tol=1e-8, precision="fp64", x0=None, maxit=None, inner_tol=None,
-              check_every=None, use_graph=True, raise_on_noconv=True):
+              check_every=None, use_graph=True, raise_on_noconv=True, method=None):
         b = L.carr(b)
         nrhs = b.size // self.n
         e = np.zeros_like(b) if x0 is None else L.carr(x0).copy()
-        o = self._opts(tol, precision, maxit, inner_tol, check_every, use_graph)
+        o = self._opts(tol, precision, maxit, inner_tol, check_every, use_graph, method)
         st = L.mfd_solve_stats()
         rc = lib().mfd_solve(self._h, ptr(b), ptr(e), nrhs, C.byref(o), C.byref(st))
         if rc != L.MFD_OK and (raise_on_noconv or rc != L.MFD_ERR_NOCONV):
@@ -464,16 +482,16 @@ class MaxwellOperator:
         return e, st
 
     def solve_dev(self, b_ptr, e_ptr, nrhs=1, tol=1e-8, precision="fp64", maxit=None, inner_tol=None,
-                  check_every=None, use_graph=True):
-        o = self._opts(tol, precision, maxit, inner_tol, check_every, use_graph)
+                  check_every=None, use_graph=True, method=None):
+        o = self._opts(tol, precision, maxit, inner_tol, check_every, use_graph, method)
         st = L.mfd_solve_stats()
         check(lib().mfd_solve_dev(self._h, C.c_void_p(b_ptr), C.c_void_p(e_ptr), nrhs, C.byref(o), C.byref(st)))
         return st
 
-    def solve_adjoint(self, g, tol=1e-8, precision="fp64", maxit=None):
+    def solve_adjoint(self, g, tol=1e-8, precision="fp64", maxit=None, method=None):
         g = L.carr(g)
         lam = np.zeros_like(g)
-        o = self._opts(tol, precision, maxit, None, None, True)
+        o = self._opts(tol, precision, maxit, None, None, True, method)
         st = L.mfd_solve_stats()
         check(lib().mfd_solve_adjoint(self._h, ptr(g), ptr(lam), C.byref(o), C.byref(st)))
         return lam, st

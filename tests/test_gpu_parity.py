@@ -37,6 +37,9 @@ CASES = {
     "cmplast": dict(N=(4, 3, 5), npml=((1, 1, 1), (1, 1, 1)), om=0.8, bloch=(False,) * 3, cmpfirst=False),
     "boundft_HH": dict(N=(4, 4, 4), npml=((1, 1, 1), (1, 1, 1)), om=0.8, bloch=(False, True, False),
                        bft=(0, 1, 0)),
+    "wide": dict(N=(70, 5, 7), npml=((3, 1, 1), (2, 1, 2)), om=0.45, bloch=(False,) * 3),
+    "wide_bloch": dict(N=(67, 3, 5), npml=((0, 0, 1), (0, 0, 1)), om=0.5, bloch=(True, True, False),
+                       kb=(0.02, 0.3, 0.0)),
 }
 
 
@@ -184,7 +187,28 @@ def test_matparams(nonuni, pkg, O):
     assert np.abs(ea[..., 0, 1]).max() > 0.1
 
 
-OPCASES = ["uniform_pml", "nonuniform_bloch", "mixed_pml_bloch", "tiny_bloch_axes", "cmplast"]
+OPCASES = ["uniform_pml", "nonuniform_bloch", "mixed_pml_bloch", "tiny_bloch_axes", "cmplast", "wide"]
+
+
+@pytest.mark.parametrize("case", ["wide", "wide_bloch", "nonuniform_bloch", "tiny_bloch_axes"])
+@pytest.mark.parametrize("ty,kz", [(1, 1), (1, 4), (2, 3), (2, 16), (4, 2), (4, 16), (0, 1)])
+def test_apply_tile_shapes(case, ty, kz, pkg, O):
+    """Every tile shape of the stencil (and the direct kernel, TY = 0) gives the CSC product."""
+    m = make(pkg, case)
+    eps, muu = rand_params(m, 8)
+    om = 0.55
+    say("tiles", case, ty, kz)
+    A = O.to_scipy(O.create_A(spec_of(m), om, eps, muu))
+    op = pkg.MaxwellOperator(m, om, εarr=eps, μarr=muu)
+    op.set_tile("fp64", ty, kz)
+    rng = np.random.default_rng(4)
+    n = 3 * m.grid.M
+    X = rng.normal(size=(2, n)) + 1j * rng.normal(size=(2, n))
+    Y = op.apply(X)
+    for r in range(2):
+        ref = A @ X[r]
+        assert np.linalg.norm(Y[r] - ref) <= 1e-12 * np.linalg.norm(ref)
+    op.close()
 
 
 @pytest.mark.parametrize("case", OPCASES)
