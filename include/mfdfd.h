@@ -72,7 +72,7 @@ typedef struct {
     double tol;               /* relative residual ||b-Ae||/||b|| in FP64 (1e-8) */
     int64_t maxit;            /* total Krylov iterations allowed per RHS */
     int32_t precision;        /* MFD_FP64 or MFD_MIXED (FP32 inner, FP64 refinement) */
-    int32_t method;           /* MFD_SQMR, MFD_COCG, or -1: SQMR for FP64, COCG for mixed */
+    int32_t method;           /* MFD_SQMR, MFD_COCG, or -1 (SQMR) */
     double inner_tol;         /* mixed: tightest inner reduction per FP32 cycle (1e-5) */
     int32_t check_every;      /* iterations between host convergence checks (32) */
     int32_t use_graph;        /* capture a chunk of iterations in a hipGraph */
@@ -84,6 +84,7 @@ typedef struct {
     double relres;            /* max over RHS of the final FP64 relative residual */
     double seconds;           /* device time of the solve */
     int32_t converged;
+    int64_t iterations_fp32;  /* of `iterations`, those run in complex-FP32 inner cycles */
 } mfd_solve_stats;
 
 const char *mfd_version(void);
@@ -160,6 +161,11 @@ int mfd_op_set_tile(mfd_op *op, int prec, int ty, int kz);
 /* Average device time (ms) of one apply of the solver's stencil kernel
  * (MFD_FP64: A; MFD_MIXED: the complex-FP32 inner operator) */
 int mfd_op_bench_apply(mfd_op *op, int64_t nrhs, int prec, int iters, double *ms_per_apply);
+/* Average device time (ms) of each kernel of one SQMR iteration, in launch
+ * order: ms[0] stencil with the qᵀPq epilogue, ms[1] r update with its
+ * reductions, ms[2] d/z/q update.  Uses (and scribbles over) the solver
+ * workspace.  No reference counterpart. */
+int mfd_op_bench_iteration(mfd_op *op, int64_t nrhs, int prec, int iters, double ms[3]);
 
 /* Solve A e = b (the user's `A \ b`); e holds the initial guess on entry. */
 void mfd_solve_opts_default(mfd_solve_opts *o);
@@ -168,6 +174,17 @@ int mfd_solve(mfd_op *op, const mfd_c64 *b, mfd_c64 *e, int64_t nrhs, const mfd_
 int mfd_solve_dev(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_solve_opts *o,
                   mfd_solve_stats *st);
 /* Adjoint solve Aᵀ λ = g using Aᵀ = S_E A S_E⁻¹ (complex symmetry). */
+/* Solve `count` independent systems A_p e_p = b_p, one RHS each (every
+ * problem keeps its own grid, ε and ω; all operators share one context).
+ * Each chunk of iterations of all unconverged problems is one hipGraph launch
+ * whose per-problem branches run concurrently.  b[p], e[p]: host arrays of
+ * 3·M_p values (e holds the initial guess); st: `count` entries or null.
+ * SQMR only.  No reference counterpart (the reference solves one `A \ b`). */
+int mfd_solve_batch(int count, mfd_op *const *ops, const mfd_c64 *const *b, mfd_c64 *const *e,
+                    const mfd_solve_opts *o, mfd_solve_stats *st);
+/* The same with device pointers on the context's GPU. */
+int mfd_solve_batch_dev(int count, mfd_op *const *ops, const void *const *b, void *const *e,
+                        const mfd_solve_opts *o, mfd_solve_stats *st);
 int mfd_solve_adjoint(mfd_op *op, const mfd_c64 *g, mfd_c64 *lam, const mfd_solve_opts *o,
                       mfd_solve_stats *st);
 /* ∂f/∂ε per E DOF for real f with Aᵀλ = (∂f/∂e)ᵀ: grad = 2ω² Re(λ ∘ e)

@@ -14,9 +14,14 @@ int op_set_graddiv_impl(mfd_op *op, int enable);
 int op_set_tile_impl(mfd_op *op, int prec, int ty, int kz);
 int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best32, double *ms_out);
 int op_bench_apply_impl(mfd_op *op, int64_t nrhs, int prec, int iters, double *ms);
+int op_bench_iteration_impl(mfd_op *op, int64_t nrhs, int prec, int iters, double *ms);
 void solve_opts_default(mfd_solve_opts *o);
 int solve_impl(mfd_op *op, const mfd_c64 *b, mfd_c64 *e, int64_t nrhs, const mfd_solve_opts *o, mfd_solve_stats *st);
 int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_solve_opts *o, mfd_solve_stats *st);
+int solve_batch_impl(int count, mfd_op *const *ops, const mfd_c64 *const *b, mfd_c64 *const *e,
+                     const mfd_solve_opts *o, mfd_solve_stats *st);
+int solve_batch_dev_impl(int count, mfd_op *const *ops, const void *const *b, void *const *e,
+                         const mfd_solve_opts *o, mfd_solve_stats *st);
 int solve_adjoint_impl(mfd_op *op, const mfd_c64 *g, mfd_c64 *lam, const mfd_solve_opts *o, mfd_solve_stats *st);
 int adjoint_grad_impl(mfd_op *op, const mfd_c64 *e, const mfd_c64 *lam, double *grad);
 int create_b_impl(mfd_ctx *ctx, const mfd::DevGrid &G, mfd_c64 omega, const mfd_c64 *mu_arr, const mfd_c64 *je,
@@ -256,6 +261,12 @@ int mfd_op_bench_apply(mfd_op *op, int64_t nrhs, int prec, int iters, double *ms
     CHECK_CTX(op->ctx);
     return op_bench_apply_impl(op, nrhs, prec, iters, ms);
 }
+
+int mfd_op_bench_iteration(mfd_op *op, int64_t nrhs, int prec, int iters, double ms[3]) {
+    if (!op || !ms) { set_error("bad argument"); return MFD_ERR_ARG; }
+    CHECK_CTX(op->ctx);
+    return op_bench_iteration_impl(op, nrhs, prec, iters, ms);
+}
 void mfd_solve_opts_default(mfd_solve_opts *o) { solve_opts_default(o); }
 int mfd_solve(mfd_op *op, const mfd_c64 *b, mfd_c64 *e, int64_t nrhs, const mfd_solve_opts *o, mfd_solve_stats *st) {
     if (!op || !b || !e) { set_error("null argument"); return MFD_ERR_ARG; }
@@ -266,6 +277,18 @@ int mfd_solve_dev(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_so
     if (!op || !b || !e) { set_error("null argument"); return MFD_ERR_ARG; }
     CHECK_CTX(op->ctx);
     return solve_dev_impl(op, b, e, nrhs, o, st);
+}
+int mfd_solve_batch(int count, mfd_op *const *ops, const mfd_c64 *const *b, mfd_c64 *const *e,
+                    const mfd_solve_opts *o, mfd_solve_stats *st) {
+    if (count < 1 || !ops || !ops[0]) { set_error("null argument"); return MFD_ERR_ARG; }
+    CHECK_CTX(ops[0]->ctx);
+    return solve_batch_impl(count, ops, b, e, o, st);
+}
+int mfd_solve_batch_dev(int count, mfd_op *const *ops, const void *const *b, void *const *e,
+                        const mfd_solve_opts *o, mfd_solve_stats *st) {
+    if (count < 1 || !ops || !ops[0]) { set_error("null argument"); return MFD_ERR_ARG; }
+    CHECK_CTX(ops[0]->ctx);
+    return solve_batch_dev_impl(count, ops, b, e, o, st);
 }
 int mfd_solve_adjoint(mfd_op *op, const mfd_c64 *g, mfd_c64 *lam, const mfd_solve_opts *o, mfd_solve_stats *st) {
     if (!op || !g || !lam) { set_error("null argument"); return MFD_ERR_ARG; }

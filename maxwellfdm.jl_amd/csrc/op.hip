@@ -669,8 +669,9 @@ static int ws_ensure(mfd_op *op, int64_t nrhs, int prec) {
     int64_t n = 3 * op->G.h.M;
     if (w.buf && w.nrhs == nrhs && w.prec == prec) return MFD_OK;
     w.release();
-    size_t es = prec == MFD_MIXED ? sizeof(c32) : sizeof(c64);
-    size_t vb = es * n * nrhs;
+    // Krylov vectors are sized for FP64 in both modes: a mixed solve whose
+    // FP32 cycles stop paying off continues with FP64 cycles in place
+    size_t vb = sizeof(c64) * n * nrhs;
     size_t align = 256;
     auto up = [&](size_t x) { return (x + align - 1) / align * align; };
     size_t bytes = 5 * up(vb) + 3 * up(sizeof(c64) * n * nrhs) + up(sizeof(c64) * (n / 3) * nrhs) +
@@ -694,7 +695,7 @@ static int ws_ensure(mfd_op *op, int64_t nrhs, int prec) {
     w.bn = (double *)p; p += up(sizeof(double) * nrhs);
     w.rn = (double *)p; p += up(sizeof(double) * nrhs);
     w.active = (int *)p; p += up(sizeof(int) * 3 * nrhs);
-    MFD_HIP(hipMemsetAsync(w.cnt, 0, sizeof(unsigned) * 3 * nrhs, op->ctx->stream));
+    MFD_HIP(hipMemsetAsync(w.cnt, 0, sizeof(unsigned) * 3 * nrhs, op->stream));
     MFD_HIP(hipHostMalloc((void **)&w.scal_h, sizeof(Scal) * nrhs, hipHostMallocDefault));
     MFD_HIP(hipHostMalloc((void **)&w.rn_h, sizeof(double) * nrhs, hipHostMallocDefault));
     w.nrhs = nrhs;
@@ -716,7 +717,7 @@ static void launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int
     const DGrid<R> &g = op->G.view<R>();
     const OpData<R> &od = opd<R>(op);
     int64_t vs = 3 * g.M;
-    hipStream_t s = op->ctx->stream;
+    hipStream_t s = op->stream;
     int ty = std::is_same<R, double>::value ? op->tile64[0] : op->tile32[0];
     int kz = std::is_same<R, double>::value ? op->tile64[1] : op->tile32[1];
     if (!op->full && !op->has_mu && ty > 0) {
@@ -749,7 +750,7 @@ template <typename R>
 static void launch_iter(mfd_op *op, int64_t nrhs) {
     Workspace &w = op->ws;
     int64_t n = w.n;
-    hipStream_t s = op->ctx->stream;
+    hipStream_t s = op->stream;
     dim3 gv(ngrid(n), (unsigned)nrhs);
     size_t pslot = (size_t)nrhs * NPART * 4;
     launch_apply<R, MODE_P, EPI_DOT>(op, (const cx<R> *)w.q, (cx<R> *)w.t, nullptr, nrhs, w.part, w.cnt, w.scal, nullptr);
@@ -770,6 +771,7 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
         if (g->boundft[a] != MFD_EE) { set_error("matrix-free operator: boundft must be EE on every axis"); return MFD_ERR_UNSUPPORTED; }
     mfd_op *op = new mfd_op();
     op->ctx = ctx;
+    op->stream = ctx->stream;
     int rc = op->G.create(g, ctx->stream);
     if (rc) { delete op; return rc; }
     hipStream_t s = ctx->stream;
@@ -851,7 +853,7 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
 
 int op_destroy_impl(mfd_op *op) {
     if (!op) return MFD_OK;
-    hipStreamSynchronize(op->ctx->stream);
+    hipStreamSynchronize(op->stream);
     op->ws.release();
     if (op->buf) hipFree(op->buf);
     if (op->ev0) hipEventDestroy(op->ev0);
@@ -866,7 +868,7 @@ int op_set_graddiv_impl(mfd_op *op, int enable) {
         return MFD_ERR_UNSUPPORTED;
     }
     op->gd = enable ? 1 : 0;
-    hipStream_t s = op->ctx->stream;
+    hipStream_t s = op->stream;
     int64_t M = op->G.h.M;
     c64 *q = (c64 *)op->d64.q;
     if (op->gd) k_jacobi_gd<<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q);
@@ -884,7 +886,7 @@ int op_apply_dev_impl(mfd_op *op, const void *x, void *y, int64_t nrhs, int prec
     if (prec == MFD_FP64) launch_apply<double, MODE_A, EPI_STORE>(op, (const c64 *)x, (c64 *)y, nullptr, nrhs, nullptr, nullptr, nullptr, nullptr);
     else launch_apply<float, MODE_A, EPI_STORE>(op, (const c32 *)x, (c32 *)y, nullptr, nrhs, nullptr, nullptr, nullptr, nullptr);
     MFD_HIP(hipGetLastError());
-    MFD_HIP(hipStreamSynchronize(op->ctx->stream));
+    MFD_HIP(hipStreamSynchronize(op->stream));
     return MFD_OK;
 }
 
@@ -894,7 +896,7 @@ int op_apply_impl(mfd_op *op, const mfd_c64 *x, mfd_c64 *y, int64_t nrhs) {
     void *dx = nullptr, *dy = nullptr;
     MFD_HIP(hipMalloc(&dx, b));
     if (hipMalloc(&dy, b) != hipSuccess) { hipFree(dx); set_error("hipMalloc failed"); return MFD_ERR_ALLOC; }
-    hipStream_t s = op->ctx->stream;
+    hipStream_t s = op->stream;
     int rc = MFD_OK;
     if (hipMemcpyAsync(dx, x, b, hipMemcpyHostToDevice, s) != hipSuccess) rc = MFD_ERR_HIP;
     if (!rc) rc = op_apply_dev_impl(op, dx, dy, nrhs, MFD_FP64);
@@ -911,7 +913,7 @@ int op_bench_apply_impl(mfd_op *op, int64_t nrhs, int prec, int iters, double *m
     void *dx = nullptr, *dy = nullptr;
     MFD_HIP(hipMalloc(&dx, es * n * nrhs));
     if (hipMalloc(&dy, es * n * nrhs) != hipSuccess) { hipFree(dx); return MFD_ERR_ALLOC; }
-    hipStream_t s = op->ctx->stream;
+    hipStream_t s = op->stream;
     hipMemsetAsync(dx, 0, es * n * nrhs, s);
     auto one = [&]() {
         if (prec == MFD_FP64) launch_apply<double, MODE_A, EPI_STORE>(op, (const c64 *)dx, (c64 *)dy, nullptr, nrhs, nullptr, nullptr, nullptr, nullptr);
@@ -928,6 +930,68 @@ int op_bench_apply_impl(mfd_op *op, int64_t nrhs, int prec, int iters, double *m
     hipFree(dx);
     hipFree(dy);
     if (e != hipSuccess) { set_error(hipGetErrorString(e)); return MFD_ERR_HIP; }
+    return MFD_OK;
+}
+
+// Scalars of an active iteration with α = β = 0, so that repeated launches of
+// one solver kernel leave the vectors bounded (timing only)
+__global__ void k_bench_scal(Scal *scal, int64_t nrhs) {
+    int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= nrhs) return;
+    Scal S = {};
+    S.rho_re = 1.0;
+    S.tau = S.cn = 1.0;
+    S.itmax = 1LL << 62;
+    scal[r] = S;
+}
+
+// Average duration of each kernel of one SQMR iteration (stencil with the
+// qᵀPq epilogue, r update with its reductions, d/z/q update), each timed over
+// `iters` back-to-back launches on the context stream with the solver's own
+// workspace and launch shapes.
+int op_bench_iteration_impl(mfd_op *op, int64_t nrhs, int prec, int iters, double *ms) {
+    if (nrhs < 1 || iters < 1) { set_error("nrhs and iters must be positive"); return MFD_ERR_ARG; }
+    prec = prec == MFD_MIXED ? MFD_MIXED : MFD_FP64;
+    MFD_TRY(ws_ensure(op, nrhs, prec));
+    Workspace &w = op->ws;
+    hipStream_t s = op->stream;
+    int64_t n = w.n;
+    size_t es = prec == MFD_FP64 ? sizeof(c64) : sizeof(c32);
+    // bounded nonzero contents (every byte 0x3c: ~1e-2 in FP32, ~2e-19 in FP64)
+    for (void *v : {w.r, w.q, w.t, w.d, w.z}) MFD_HIP(hipMemsetAsync(v, 0x3c, es * n * nrhs, s));
+    dim3 gv(ngrid(n), (unsigned)nrhs);
+    unsigned gs = (unsigned)((nrhs + 63) / 64);
+    auto timed = [&](int which) -> double {
+        auto one = [&]() {
+            if (which == 0) {
+                if (prec == MFD_FP64)
+                    launch_apply<double, MODE_P, EPI_DOT>(op, (const c64 *)w.q, (c64 *)w.t, nullptr, nrhs, w.part, w.cnt, w.scal, nullptr);
+                else launch_apply<float, MODE_P, EPI_DOT>(op, (const c32 *)w.q, (c32 *)w.t, nullptr, nrhs, w.part, w.cnt, w.scal, nullptr);
+            } else if (which == 1) {
+                size_t ps = (size_t)nrhs * NPART * 4;
+                if (prec == MFD_FP64) k_upd_r<double><<<gv, BS, 0, s>>>((c64 *)w.r, (const c64 *)w.t, n, n, w.part + ps, w.cnt + nrhs, w.scal);
+                else k_upd_r<float><<<gv, BS, 0, s>>>((c32 *)w.r, (const c32 *)w.t, n, n, w.part + ps, w.cnt + nrhs, w.scal);
+            } else {
+                if (prec == MFD_FP64) k_upd_dzq<double><<<gv, BS, 0, s>>>((c64 *)w.d, (c64 *)w.z, (c64 *)w.q, (const c64 *)w.r, n, n, w.scal);
+                else k_upd_dzq<float><<<gv, BS, 0, s>>>((c32 *)w.d, (c32 *)w.z, (c32 *)w.q, (const c32 *)w.r, n, n, w.scal);
+            }
+        };
+        k_bench_scal<<<gs, 64, 0, s>>>(w.scal, nrhs);
+        for (int k = 0; k < 3; ++k) one();
+        k_bench_scal<<<gs, 64, 0, s>>>(w.scal, nrhs);
+        hipEventRecord(op->ev0, s);
+        for (int it = 0; it < iters; ++it) one();
+        hipEventRecord(op->ev1, s);
+        if (hipEventSynchronize(op->ev1) != hipSuccess) return -1.0;
+        float t = 0;
+        hipEventElapsedTime(&t, op->ev0, op->ev1);
+        return t / iters;
+    };
+    for (int k = 0; k < 3; ++k) {
+        ms[k] = timed(k);
+        if (ms[k] < 0) { set_error("kernel timing failed"); return MFD_ERR_HIP; }
+    }
+    MFD_HIP(hipGetLastError());
     return MFD_OK;
 }
 
@@ -1105,7 +1169,7 @@ template <typename R>
 static void launch_cocg_iter(mfd_op *op, int64_t nrhs) {
     Workspace &w = op->ws;
     int64_t n = w.n;
-    hipStream_t s = op->ctx->stream;
+    hipStream_t s = op->stream;
     dim3 gv(ngrid(n), (unsigned)nrhs);
     size_t pslot = (size_t)nrhs * NPART * 4;
     // w.q holds p, w.t holds t = P p, w.z holds x, w.r holds r
@@ -1120,7 +1184,7 @@ template <typename R>
 static int run_cocg(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_solve_opts &o, double delta,
                     long long *iters_out, long long *updates_out, double *worst_out, bool *conv_out) {
     Workspace &w = op->ws;
-    hipStream_t s = op->ctx->stream;
+    hipStream_t s = op->stream;
     int64_t n = w.n;
     dim3 gv(ngrid(n), (unsigned)nrhs);
     size_t pslot = (size_t)nrhs * NPART * 4;
@@ -1267,7 +1331,7 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
                      long long budget, long long *iters_out, const void *b = nullptr, const void *e = nullptr,
                      const double *bn = nullptr) {
     Workspace &w = op->ws;
-    hipStream_t s = op->ctx->stream;
+    hipStream_t s = op->stream;
     int64_t n = w.n;
     dim3 gv(ngrid(n), (unsigned)nrhs);
     size_t pslot = (size_t)nrhs * NPART * 4;
@@ -1278,7 +1342,8 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
     MFD_HIP(hipGetLastError());
     int chunk = o->check_every > 0 ? o->check_every : 32;
     bool use_graph = o->use_graph != 0;
-    if (use_graph && (w.graph_iters != chunk || w.graph_kind != 1)) {
+    const int kind = sizeof(R) == 4 ? 1 : 3;
+    if (use_graph && (w.graph_iters != chunk || w.graph_kind != kind)) {
         if (w.graph) { hipGraphExecDestroy(w.graph); w.graph = nullptr; }
         hipGraph_t gr;
         MFD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1287,13 +1352,17 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
         MFD_HIP(hipGraphInstantiate(&w.graph, gr, nullptr, nullptr, 0));
         hipGraphDestroy(gr);
         w.graph_iters = chunk;
-        w.graph_kind = 1;
+        w.graph_kind = kind;
     }
     long long done_it = 0;
     const char *pe = getenv("MFD_PROBE_EVERY");
     const long long probe = std::max<long long>(chunk, pe ? atoll(pe) : 512) / chunk * chunk;
     double best_probe = 1e300;
     int stall = 0;
+    const char *pst = getenv("MFD_PROBE_STALL");
+    const int stall_max = pst ? atoi(pst) : 2;
+    const char *te = getenv("MFD_TRACE_EVERY");
+    const long long trace_every = te ? atoll(te) / chunk * chunk : 0;
     while (true) {
         if (use_graph) MFD_HIP(hipGraphLaunch(w.graph, s));
         else
@@ -1310,6 +1379,8 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
         }
         *iters_out = mx;
         if (all || mx >= budget) break;
+        if (trace_every > 0 && done_it % trace_every == 0)
+            fprintf(stderr, "[mfd iter] it %lld tau/cn %.3e\n", mx, w.scal_h[0].tau / w.scal_h[0].cn);
         // FP32 cycles: probe the true FP64 residual every `probe` iterations and
         // end the cycle once it stops improving (2x per probe window)
         if (sizeof(R) == 4 && b && e && done_it % probe == 0) {
@@ -1324,7 +1395,7 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
             for (int64_t r = 0; r < nrhs; ++r) worst = std::max(worst, bn[r] > 0 ? sqrt(w.rn_h[r] / bn[r]) : 0.0);
             if (getenv("MFD_SOLVE_TRACE")) fprintf(stderr, "[mfd probe] it %lld true residual %.3e\n", mx, worst);
             if (worst > 0.5 * best_probe) {
-                if (++stall >= 2) break;
+                if (++stall >= stall_max) break;
             } else stall = 0;
             best_probe = std::min(best_probe, worst);
         }
@@ -1340,12 +1411,12 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
     int prec = o.precision == MFD_MIXED ? MFD_MIXED : MFD_FP64;
     MFD_TRY(ws_ensure(op, nrhs, prec));
     Workspace &w = op->ws;
-    hipStream_t s = op->ctx->stream;
+    hipStream_t s = op->stream;
     int64_t n = w.n;
     size_t pslot = (size_t)nrhs * NPART * 4;
     dim3 gv(ngrid(n), (unsigned)nrhs);
     hipEventRecord(op->ev0, s);
-    int method = o.method >= 0 ? o.method : (prec == MFD_MIXED ? MFD_COCG : MFD_SQMR);
+    int method = o.method >= 0 ? o.method : MFD_SQMR;
     if (method == MFD_COCG) {
         long long it = 0, upd = 0;
         double worst = 0;
@@ -1366,6 +1437,7 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
             st->relres = worst;
             st->seconds = ms * 1e-3;
             st->converged = conv;
+            st->iterations_fp32 = prec == MFD_MIXED ? it : 0;
         }
         if (!conv) { set_error("solve did not reach the tolerance within maxit"); return MFD_ERR_NOCONV; }
         return MFD_OK;
@@ -1375,9 +1447,12 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
     MFD_HIP(hipMemcpyAsync(bn.data(), w.bn, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
     std::vector<int> active(nrhs, 1);
     int *active_d = w.active;
-    long long total_it = 0, cycles = 0;
-    double worst = 0;
+    long long total_it = 0, cycles = 0, it32 = 0;
+    double worst = 0, prev_worst = 0;
     bool conv = false;
+    // FP32 cycles run while each gains at least 10x on the FP64 residual; a
+    // cycle that gains less switches the rest of the solve to FP64 cycles
+    int cyc_prec = prec;
     while (true) {
         launch_apply<double, MODE_A, EPI_RESID>(op, (const c64 *)e, (c64 *)w.r64, (const c64 *)b, nrhs,
                                                 w.part + 2 * pslot, w.cnt + 2 * nrhs, nullptr, w.rn);
@@ -1394,6 +1469,8 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
         }
         if (all) { conv = true; break; }
         if (total_it >= o.maxit) break;
+        if (cyc_prec == MFD_MIXED && cycles > 0 && worst > 0.1 * prev_worst) cyc_prec = MFD_FP64;
+        prev_worst = worst;
         MFD_HIP(hipMemcpyAsync(active_d, active.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s));
         if (op->gd) {
             // the inner system is A' z = r' with r' = r + (1/ω²) ε G W Div r
@@ -1406,23 +1483,24 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
         // (FP32 iterations stagnate below that)
         double need = std::max(0.1 * o.tol / std::max(worst, 1e-300), 1e-15);
         double target = std::min(need, 1e-2);
-        if (prec == MFD_MIXED) target = std::max(target, o.inner_tol);
+        if (cyc_prec == MFD_MIXED) target = std::max(target, o.inner_tol);
         long long it = 0;
         int rc;
-        if (prec == MFD_MIXED)
+        if (cyc_prec == MFD_MIXED)
             rc = run_inner<float>(op, nrhs, &o, target, active_d, o.maxit - total_it, &it, b, e, bn.data());
         else rc = run_inner<double>(op, nrhs, &o, target, active_d, o.maxit - total_it, &it);
         if (rc) return rc;
         total_it += it;
+        if (cyc_prec == MFD_MIXED) it32 += it;
         ++cycles;
         if (getenv("MFD_SOLVE_TRACE")) {
             MFD_HIP(hipMemcpyAsync(w.scal_h, w.scal, sizeof(Scal) * nrhs, hipMemcpyDeviceToHost, s));
             MFD_HIP(hipStreamSynchronize(s));
             fprintf(stderr, "[mfd solve] cycle %lld prec %s residual %.3e target %.2e iterations %lld tau/cn %.3e stag %d\n",
-                    cycles, prec == MFD_MIXED ? "fp32" : "fp64", worst, target, it,
+                    cycles, cyc_prec == MFD_MIXED ? "fp32" : "fp64", worst, target, it,
                     w.scal_h[0].cn > 0 ? w.scal_h[0].tau / w.scal_h[0].cn : 0.0, w.scal_h[0].stag);
         }
-        if (prec == MFD_MIXED) k_upd_e<float><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (c64 *)e, (const c32 *)w.z, n, n);
+        if (cyc_prec == MFD_MIXED) k_upd_e<float><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (c64 *)e, (const c32 *)w.z, n, n);
         else k_upd_e<double><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (c64 *)e, (const c64 *)w.z, n, n);
         MFD_HIP(hipGetLastError());
         if (cycles > 200) break;
@@ -1437,9 +1515,290 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
         st->relres = worst;
         st->seconds = ms * 1e-3;
         st->converged = conv;
+        st->iterations_fp32 = it32;
     }
     if (!conv) { set_error("solve did not reach the tolerance within maxit"); return MFD_ERR_NOCONV; }
     return MFD_OK;
+}
+
+// copies each problem's solver scalars into one array (one readback per chunk)
+__global__ void k_gather_scal(Scal *const *src, Scal *dst, int count) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < count) dst[p] = *src[p];
+}
+
+namespace {
+struct BatchProb {
+    mfd_op *op;
+    const void *b;
+    void *e;
+    int prec;                  // precision of the current cycle
+    double bn = 0, worst = 1, prev = 0, best_probe = 1e300;
+    int stall = 0;
+    long long it_total = 0, it32 = 0, cycles = 0, next_probe = 0;
+    bool conv = false, done = false;
+};
+}  // namespace
+
+// Independent problems (own grid, ε, ω; one RHS each) solved in lockstep: each
+// chunk of SQMR iterations of every live problem is one hipGraph launch whose
+// per-problem branches run on separate streams; between chunks the host reads
+// all solver scalars in one copy and runs the FP64 refinement step of each
+// problem whose inner cycle ended (same rules as solve_dev_impl's SQMR path).
+int solve_batch_dev_impl(int count, mfd_op *const *ops, const void *const *b, void *const *e,
+                         const mfd_solve_opts *oin, mfd_solve_stats *st) {
+    if (count < 1 || count > 1024 || !ops || !b || !e) { set_error("count must be in [1, 1024] with non-null arrays"); return MFD_ERR_ARG; }
+    mfd_solve_opts o;
+    if (oin) o = *oin; else solve_opts_default(&o);
+    if (o.method == MFD_COCG) { set_error("batched solve: only the SQMR method is implemented"); return MFD_ERR_UNSUPPORTED; }
+    mfd_ctx *ctx = ops[0] ? ops[0]->ctx : nullptr;
+    for (int p = 0; p < count; ++p) {
+        if (!ops[p] || ops[p]->ctx != ctx || !b[p] || !e[p]) { set_error("batched solve: every problem needs an operator of the same context and device b, e"); return MFD_ERR_ARG; }
+        for (int q = 0; q < p; ++q)
+            if (ops[q] == ops[p]) { set_error("batched solve: an operator appears twice"); return MFD_ERR_ARG; }
+    }
+    const int prec0 = o.precision == MFD_MIXED ? MFD_MIXED : MFD_FP64;
+    hipStream_t s = ctx->stream;
+    std::vector<BatchProb> P(count);
+    for (int p = 0; p < count; ++p) {
+        P[p].op = ops[p];
+        P[p].b = b[p];
+        P[p].e = e[p];
+        P[p].prec = prec0;
+        MFD_TRY(ws_ensure(ops[p], 1, prec0));
+    }
+    const int nstreams = std::min(count, 32);
+    std::vector<hipStream_t> streams(nstreams, nullptr);
+    std::vector<hipEvent_t> joins(nstreams, nullptr);
+    hipEvent_t fork = nullptr, tev0 = nullptr, tev1 = nullptr;
+    Scal **src_d = nullptr;
+    Scal *all_d = nullptr, *all_h = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int rc = MFD_OK;
+    auto fail = [&](hipError_t err) { set_error(hipGetErrorString(err)); rc = MFD_ERR_HIP; };
+#define BCHK(x) do { hipError_t err_ = (x); if (err_ != hipSuccess) { fail(err_); goto cleanup; } } while (0)
+    {
+        for (int i = 0; i < nstreams; ++i) {
+            BCHK(hipStreamCreateWithFlags(&streams[i], hipStreamNonBlocking));
+            BCHK(hipEventCreateWithFlags(&joins[i], hipEventDisableTiming));
+        }
+        BCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        BCHK(hipEventCreate(&tev0));
+        BCHK(hipEventCreate(&tev1));
+        BCHK(hipMalloc(&src_d, sizeof(Scal *) * count));
+        BCHK(hipMalloc(&all_d, sizeof(Scal) * count));
+        BCHK(hipHostMalloc((void **)&all_h, sizeof(Scal) * count, hipHostMallocDefault));
+        {
+            std::vector<Scal *> src(count);
+            for (int p = 0; p < count; ++p) src[p] = ops[p]->ws.scal;
+            BCHK(hipMemcpyAsync(src_d, src.data(), sizeof(Scal *) * count, hipMemcpyHostToDevice, s));
+            BCHK(hipStreamSynchronize(s));
+        }
+        hipEventRecord(tev0, s);
+        for (int p = 0; p < count; ++p) {
+            Workspace &w = ops[p]->ws;
+            dim3 gv(ngrid(w.n), 1);
+            k_norm2<<<gv, BS, 0, s>>>((const c64 *)b[p], w.n, w.n, w.part + 2 * NPART * 4, w.cnt + 2, w.bn);
+        }
+        for (int p = 0; p < count; ++p) {
+            BCHK(hipMemcpyAsync(ops[p]->ws.rn_h, ops[p]->ws.bn, sizeof(double), hipMemcpyDeviceToHost, s));
+            BCHK(hipStreamSynchronize(s));
+            P[p].bn = ops[p]->ws.rn_h[0];
+        }
+        const int chunk = o.check_every > 0 ? o.check_every : 32;
+        const long long probe_every = std::max<long long>(chunk, 512) / chunk * chunk;
+        // FP64 residual of problem p; ends it or starts its next inner cycle
+        auto start_cycle = [&](BatchProb &B) -> int {
+            mfd_op *op = B.op;
+            Workspace &w = op->ws;
+            int64_t n = w.n;
+            dim3 gv(ngrid(n), 1);
+            launch_apply<double, MODE_A, EPI_RESID>(op, (const c64 *)B.e, (c64 *)w.r64, (const c64 *)B.b, 1,
+                                                    w.part + 2 * NPART * 4, w.cnt + 2, nullptr, w.rn);
+            MFD_HIP(hipMemcpyAsync(w.rn_h, w.rn, sizeof(double), hipMemcpyDeviceToHost, s));
+            MFD_HIP(hipStreamSynchronize(s));
+            double rel = B.bn > 0 ? sqrt(w.rn_h[0] / B.bn) : 0.0;
+            B.worst = rel;
+            if (rel <= o.tol) { B.conv = B.done = true; return MFD_OK; }
+            if (B.it_total >= o.maxit || B.cycles > 200) { B.done = true; return MFD_OK; }
+            if (B.prec == MFD_MIXED && B.cycles > 0 && rel > 0.1 * B.prev) B.prec = MFD_FP64;
+            B.prev = rel;
+            if (op->gd) {
+                k_gd_psi<<<gv, BS, 0, s>>>(op->G.d64, op->d64.wnode, (const c64 *)w.r64, (c64 *)w.psi, n);
+                k_gd_rhs<<<gv, BS, 0, s>>>(op->G.d64, op->d64, (const c64 *)w.psi, (c64 *)w.r64, n, cinv(op->d64.om2));
+            }
+            double need = std::max(0.1 * o.tol / std::max(rel, 1e-300), 1e-15);
+            double target = std::min(need, 1e-2);
+            if (B.prec == MFD_MIXED) target = std::max(target, o.inner_tol);
+            const int one = 1;
+            MFD_HIP(hipMemcpyAsync(w.active, &one, sizeof(int), hipMemcpyHostToDevice, s));
+            long long budget = o.maxit - B.it_total;
+            if (B.prec == MFD_MIXED)
+                k_init<float><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (const c64 *)w.r64, (c32 *)w.r, (c32 *)w.q, (c32 *)w.d,
+                                                (c32 *)w.z, n, n, w.part + 2 * NPART * 4, w.cnt + 2, w.scal, target, w.active, budget);
+            else
+                k_init<double><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (const c64 *)w.r64, (c64 *)w.r, (c64 *)w.q, (c64 *)w.d,
+                                                 (c64 *)w.z, n, n, w.part + 2 * NPART * 4, w.cnt + 2, w.scal, target, w.active, budget);
+            MFD_HIP(hipGetLastError());
+            MFD_HIP(hipStreamSynchronize(s));   // the pageable `one` above
+            B.cycles += 1;
+            B.best_probe = 1e300;
+            B.stall = 0;
+            B.next_probe = probe_every;
+            return MFD_OK;
+        };
+        // end of an inner cycle: e += (Q T⁻¹) z in FP64
+        auto end_cycle = [&](BatchProb &B, long long it) {
+            Workspace &w = B.op->ws;
+            dim3 gv(ngrid(w.n), 1);
+            if (B.prec == MFD_MIXED) k_upd_e<float><<<gv, BS, 0, s>>>(B.op->G.d64, B.op->d64, (c64 *)B.e, (const c32 *)w.z, w.n, w.n);
+            else k_upd_e<double><<<gv, BS, 0, s>>>(B.op->G.d64, B.op->d64, (c64 *)B.e, (const c64 *)w.z, w.n, w.n);
+            B.it_total += it;
+            if (B.prec == MFD_MIXED) B.it32 += it;
+        };
+        for (int p = 0; p < count; ++p) {
+            int r = start_cycle(P[p]);
+            if (r) { rc = r; goto cleanup; }
+        }
+        std::vector<int> cap_prec;   // precision each problem was captured with (-1: not in the graph)
+        int cap_live = 0;
+        while (true) {
+            int live = 0;
+            bool recap = gexec == nullptr;
+            for (int p = 0; p < count; ++p) {
+                if (P[p].done) continue;
+                ++live;
+                if (cap_prec.empty() || cap_prec[p] != P[p].prec) recap = true;
+            }
+            if (live == 0) break;
+            if (live < (3 * cap_live) / 4) recap = true;
+            if (recap) {
+                if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
+                cap_prec.assign(count, -1);
+                hipGraph_t gr;
+                BCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                hipEventRecord(fork, s);
+                int k = 0;
+                for (int p = 0; p < count; ++p) {
+                    if (P[p].done) continue;
+                    hipStream_t sp = streams[k % nstreams];
+                    if (k < nstreams) hipStreamWaitEvent(sp, fork, 0);
+                    ++k;
+                    P[p].op->stream = sp;
+                    for (int it = 0; it < chunk; ++it) {
+                        if (P[p].prec == MFD_MIXED) launch_iter<float>(P[p].op, 1);
+                        else launch_iter<double>(P[p].op, 1);
+                    }
+                    P[p].op->stream = s;
+                    cap_prec[p] = P[p].prec;
+                }
+                for (int i = 0; i < std::min(k, nstreams); ++i) {
+                    hipEventRecord(joins[i], streams[i]);
+                    hipStreamWaitEvent(s, joins[i], 0);
+                }
+                hipError_t ce = hipStreamEndCapture(s, &gr);
+                BCHK(ce);
+                ce = hipGraphInstantiate(&gexec, gr, nullptr, nullptr, 0);
+                hipGraphDestroy(gr);
+                BCHK(ce);
+                cap_live = live;
+            }
+            BCHK(hipGraphLaunch(gexec, s));
+            k_gather_scal<<<(count + 63) / 64, 64, 0, s>>>(src_d, all_d, count);
+            BCHK(hipMemcpyAsync(all_h, all_d, sizeof(Scal) * count, hipMemcpyDeviceToHost, s));
+            BCHK(hipStreamSynchronize(s));
+            for (int p = 0; p < count; ++p) {
+                BatchProb &B = P[p];
+                if (B.done || cap_prec[p] != B.prec) continue;
+                const Scal &S = all_h[p];
+                bool end = S.conv != 0;
+                if (!end && B.prec == MFD_MIXED && S.it >= B.next_probe) {
+                    // true FP64 residual of e + (Q T⁻¹) z: the FP32 cycle ends
+                    // once two probes in a row fail to halve it
+                    B.next_probe += probe_every;
+                    Workspace &w = B.op->ws;
+                    dim3 gv(ngrid(w.n), 1);
+                    k_probe_x<float><<<gv, BS, 0, s>>>(B.op->G.d64, B.op->d64, (const c64 *)B.e, (const c32 *)w.z, (c64 *)w.et, w.n, w.n);
+                    launch_apply<double, MODE_A, EPI_RESID>(B.op, (const c64 *)w.et, (c64 *)w.et2, (const c64 *)B.b, 1,
+                                                            w.part + 2 * NPART * 4, w.cnt + 2, nullptr, w.rn);
+                    BCHK(hipMemcpyAsync(w.rn_h, w.rn, sizeof(double), hipMemcpyDeviceToHost, s));
+                    BCHK(hipStreamSynchronize(s));
+                    double pr = B.bn > 0 ? sqrt(w.rn_h[0] / B.bn) : 0.0;
+                    if (pr > 0.5 * B.best_probe) end = ++B.stall >= 2;
+                    else B.stall = 0;
+                    B.best_probe = std::min(B.best_probe, pr);
+                }
+                if (end) {
+                    end_cycle(B, S.it);
+                    int r = start_cycle(B);
+                    if (r) { rc = r; goto cleanup; }
+                }
+            }
+        }
+        hipEventRecord(tev1, s);
+        BCHK(hipEventSynchronize(tev1));
+        float ms = 0;
+        hipEventElapsedTime(&ms, tev0, tev1);
+        bool all = true;
+        for (int p = 0; p < count; ++p) {
+            if (!P[p].conv) all = false;
+            if (st) {
+                st[p].iterations = P[p].it_total;
+                st[p].outer_cycles = P[p].cycles;
+                st[p].relres = P[p].worst;
+                st[p].seconds = ms * 1e-3;
+                st[p].converged = P[p].conv;
+                st[p].iterations_fp32 = P[p].it32;
+            }
+        }
+        if (!all) { set_error("batched solve: some problems did not reach the tolerance within maxit"); rc = MFD_ERR_NOCONV; }
+    }
+cleanup:
+#undef BCHK
+    for (int p = 0; p < count; ++p) ops[p]->stream = s;
+    hipStreamSynchronize(s);
+    if (gexec) hipGraphExecDestroy(gexec);
+    for (int i = 0; i < nstreams; ++i) {
+        if (streams[i]) hipStreamDestroy(streams[i]);
+        if (joins[i]) hipEventDestroy(joins[i]);
+    }
+    if (fork) hipEventDestroy(fork);
+    if (tev0) hipEventDestroy(tev0);
+    if (tev1) hipEventDestroy(tev1);
+    if (src_d) hipFree(src_d);
+    if (all_d) hipFree(all_d);
+    if (all_h) hipHostFree(all_h);
+    return rc;
+}
+
+// Host-pointer form of the batched solve: b[p], e[p] are host arrays of 3M_p values.
+int solve_batch_impl(int count, mfd_op *const *ops, const mfd_c64 *const *b, mfd_c64 *const *e,
+                     const mfd_solve_opts *o, mfd_solve_stats *st) {
+    if (count < 1 || count > 1024 || !ops || !b || !e) { set_error("count must be in [1, 1024] with non-null arrays"); return MFD_ERR_ARG; }
+    for (int p = 0; p < count; ++p)
+        if (!ops[p] || !b[p] || !e[p]) { set_error("batched solve: null operator or array"); return MFD_ERR_ARG; }
+    hipStream_t s = ops[0]->ctx->stream;
+    std::vector<void *> db(count, nullptr), de(count, nullptr);
+    int rc = MFD_OK;
+    for (int p = 0; p < count && !rc; ++p) {
+        size_t bytes = sizeof(c64) * 3 * ops[p]->G.h.M;
+        if (hipMalloc(&db[p], bytes) != hipSuccess || hipMalloc(&de[p], bytes) != hipSuccess) { set_error("hipMalloc failed"); rc = MFD_ERR_ALLOC; break; }
+        if (hipMemcpyAsync(db[p], b[p], bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(de[p], e[p], bytes, hipMemcpyHostToDevice, s) != hipSuccess) { set_error("copy to device failed"); rc = MFD_ERR_HIP; }
+    }
+    if (!rc) rc = solve_batch_dev_impl(count, ops, db.data(), de.data(), o, st);
+    if (rc == MFD_OK || rc == MFD_ERR_NOCONV) {
+        for (int p = 0; p < count; ++p) {
+            size_t bytes = sizeof(c64) * 3 * ops[p]->G.h.M;
+            if (hipMemcpyAsync(e[p], de[p], bytes, hipMemcpyDeviceToHost, s) != hipSuccess) rc = MFD_ERR_HIP;
+        }
+        if (hipStreamSynchronize(s) != hipSuccess) rc = MFD_ERR_HIP;
+    }
+    for (int p = 0; p < count; ++p) {
+        if (db[p]) hipFree(db[p]);
+        if (de[p]) hipFree(de[p]);
+    }
+    return rc;
 }
 
 int solve_impl(mfd_op *op, const mfd_c64 *b, mfd_c64 *e, int64_t nrhs, const mfd_solve_opts *o, mfd_solve_stats *st) {
@@ -1448,7 +1807,7 @@ int solve_impl(mfd_op *op, const mfd_c64 *b, mfd_c64 *e, int64_t nrhs, const mfd
     void *db = nullptr, *de = nullptr;
     MFD_HIP(hipMalloc(&db, bytes));
     if (hipMalloc(&de, bytes) != hipSuccess) { hipFree(db); set_error("hipMalloc failed"); return MFD_ERR_ALLOC; }
-    hipStream_t s = op->ctx->stream;
+    hipStream_t s = op->stream;
     int rc = MFD_OK;
     if (hipMemcpyAsync(db, b, bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(de, e, bytes, hipMemcpyHostToDevice, s) != hipSuccess)
@@ -1469,7 +1828,7 @@ int solve_adjoint_impl(mfd_op *op, const mfd_c64 *gvec, mfd_c64 *lam, const mfd_
     void *db = nullptr, *de = nullptr;
     MFD_HIP(hipMalloc(&db, bytes));
     if (hipMalloc(&de, bytes) != hipSuccess) { hipFree(db); return MFD_ERR_ALLOC; }
-    hipStream_t s = op->ctx->stream;
+    hipStream_t s = op->stream;
     int rc = MFD_OK;
     // Aᵀ λ = g  <=>  A (S⁻¹ λ) = S⁻¹ g
     if (hipMemcpyAsync(db, gvec, bytes, hipMemcpyHostToDevice, s) != hipSuccess || hipMemsetAsync(de, 0, bytes, s) != hipSuccess)
@@ -1492,7 +1851,7 @@ int adjoint_grad_impl(mfd_op *op, const mfd_c64 *e, const mfd_c64 *lam, double *
     int64_t n = 3 * op->G.h.M;
     void *d = nullptr;
     MFD_HIP(hipMalloc(&d, (2 * sizeof(c64) + sizeof(double)) * n));
-    hipStream_t s = op->ctx->stream;
+    hipStream_t s = op->stream;
     c64 *de = (c64 *)d, *dl = de + n;
     double *dg = (double *)(dl + n);
     int rc = MFD_OK;

@@ -56,6 +56,7 @@ struct Workspace {
 
 struct mfd_op {
     mfd_ctx *ctx;
+    hipStream_t stream = nullptr;   // the context's stream; a batched solve re-points it while capturing
     mfd::DevGrid G;
     mfd::c64 omega;
     int full = 0, has_mu = 0;

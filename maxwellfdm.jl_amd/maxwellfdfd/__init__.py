@@ -11,6 +11,6 @@ from .core import (DUAL, EE, HH, PRIM, Ball, Box, Cuboid, Grid, MaxwellOperator,
                    create_A, create_b, create_curls, create_eikL, create_field_array, create_linsys,
                    create_param_array, create_paramops, create_srcs, create_stretched_dls, curl_csc,
                    distweights, field_arr2vec, nX, nY, nZ, paramop_csc, set_boundft, set_kbloch, set_Npml,
-                   set_omegapml, set_ωpml, to_scipy)
+                   set_omegapml, set_ωpml, solve_batch, to_scipy)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -40,7 +40,7 @@ class mfd_solve_opts(C.Structure):
 
 class mfd_solve_stats(C.Structure):
     _fields_ = [("iterations", C.c_int64), ("outer_cycles", C.c_int64), ("relres", C.c_double),
-                ("seconds", C.c_double), ("converged", C.c_int32)]
+                ("seconds", C.c_double), ("converged", C.c_int32), ("iterations_fp32", C.c_int64)]
 
 
 class MfdError(RuntimeError):
@@ -93,9 +93,12 @@ SIGNATURES = {
     "mfd_op_autotune": (C.c_int, [_P, C.c_int64, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                   C.POINTER(C.c_double)]),
     "mfd_op_bench_apply": (C.c_int, [_P, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "mfd_op_bench_iteration": (C.c_int, [_P, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "mfd_solve_opts_default": (None, [C.POINTER(mfd_solve_opts)]),
     "mfd_solve": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(mfd_solve_opts), C.POINTER(mfd_solve_stats)]),
     "mfd_solve_dev": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(mfd_solve_opts), C.POINTER(mfd_solve_stats)]),
+    "mfd_solve_batch": (C.c_int, [C.c_int, _P, _P, _P, C.POINTER(mfd_solve_opts), _P]),
+    "mfd_solve_batch_dev": (C.c_int, [C.c_int, _P, _P, _P, C.POINTER(mfd_solve_opts), _P]),
     "mfd_solve_adjoint": (C.c_int, [_P, _P, _P, C.POINTER(mfd_solve_opts), C.POINTER(mfd_solve_stats)]),
     "mfd_adjoint_grad": (C.c_int, [_P, _P, _P, _P]),
 }

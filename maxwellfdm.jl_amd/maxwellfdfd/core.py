@@ -389,6 +389,39 @@ def create_linsys(ft, ω, mdl, jₑ=None, jₘ=None):
 
 # ------------------------------------------------------- matrix-free solve
 
+def _solve_batch(ops, bs, es, dev, o, raise_on_noconv=True):
+    count = len(ops)
+    if not (count == len(bs) == len(es)):
+        raise MfdArgumentError("solve_batch: one b and one e per operator")
+    hs = (C.c_void_p * count)(*[op._h for op in ops])
+    if dev:
+        bp = (C.c_void_p * count)(*bs)
+        ep = (C.c_void_p * count)(*es)
+    else:
+        bp = (C.c_void_p * count)(*[ptr(b) for b in bs])
+        ep = (C.c_void_p * count)(*[ptr(e) for e in es])
+    st = (L.mfd_solve_stats * count)()
+    fn = lib().mfd_solve_batch_dev if dev else lib().mfd_solve_batch
+    rc = fn(count, hs, bp, ep, C.byref(o), st)
+    if rc != L.MFD_OK and (raise_on_noconv or rc != L.MFD_ERR_NOCONV):
+        check(rc)
+    return list(st)
+
+
+def solve_batch(ops, bs, tol=1e-8, precision="fp64", maxit=None, inner_tol=None, check_every=None,
+                use_graph=True, raise_on_noconv=True):
+    """Solve independent systems ops[p] e_p = bs[p] (each its own grid, ε, ω)
+    in one lockstep device loop; returns ([e_p], [stats_p])."""
+    bs = [L.carr(b) for b in bs]
+    for op, b in zip(ops, bs):
+        if b.size != op.n:
+            raise MfdArgumentError("solve_batch: b has the wrong length for its operator")
+    es = [np.zeros_like(b) for b in bs]
+    o = MaxwellOperator._opts(tol, precision, maxit, inner_tol, check_every, use_graph)
+    st = _solve_batch(list(ops), bs, es, False, o, raise_on_noconv)
+    return es, st
+
+
 class MaxwellOperator:
     """Device-resident A = Cₘ μ⁻¹ Cₑ − ω² Pε (model.jl:235-237) with its solver.
 
@@ -452,6 +485,14 @@ class MaxwellOperator:
                                        C.byref(ms)))
         return ms.value
 
+    def bench_iteration(self, nrhs=1, precision="fp64", iters=20):
+        """Average ms of the three kernels of one solver iteration:
+        (stencil with qᵀPq, r update, d/z/q update).  Overwrites the solver workspace."""
+        ms = (C.c_double * 3)()
+        check(lib().mfd_op_bench_iteration(self._h, nrhs, L.MFD_FP64 if precision == "fp64" else L.MFD_MIXED,
+                                           iters, ms))
+        return tuple(ms)
+
     @staticmethod
     def _opts(tol, precision, maxit, inner_tol, check_every, use_graph, method=None):
         o = L.mfd_solve_opts()
@@ -487,6 +528,12 @@ class MaxwellOperator:
         st = L.mfd_solve_stats()
         check(lib().mfd_solve_dev(self._h, C.c_void_p(b_ptr), C.c_void_p(e_ptr), nrhs, C.byref(o), C.byref(st)))
         return st
+
+    def solve_batch_dev(self, others, b_ptrs, e_ptrs, tol=1e-8, precision="fp64", maxit=None, inner_tol=None,
+                        check_every=None, use_graph=True):
+        """Device-pointer form of `solve_batch` with this operator first."""
+        return _solve_batch([self] + list(others), b_ptrs, e_ptrs, True,
+                            self._opts(tol, precision, maxit, inner_tol, check_every, use_graph))
 
     def solve_adjoint(self, g, tol=1e-8, precision="fp64", maxit=None, method=None):
         g = L.carr(g)

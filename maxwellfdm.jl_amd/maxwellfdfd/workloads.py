@@ -72,3 +72,17 @@ def dipole_rhs(mdl, omega, dipoles):
         je, jm = M.create_srcs(mdl)
         bs.append(M.create_b(M.EE, omega, mdl, je, jm))
     return np.array(bs)
+
+
+def uneven_problems(count=32, nmin=48, nmax=128, npml=10, nbox=40, seed=7):
+    """Batch of independent problems: cubic grids N ~ U{nmin..nmax}, each with
+    its own seeded random-box ε (as `random_boxes`), one random dipole and
+    ω ~ U[2π/30, 2π/16].  Returns [(model, ω)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for p in range(count):
+        N = int(rng.integers(nmin, nmax + 1))
+        om = float(rng.uniform(2 * np.pi / 30, 2 * np.pi / 16))
+        m, _ = random_boxes(N=N, npml=npml, nbox=nbox, lam=2 * np.pi / om, seed=seed * 1000 + p, ndip=1)
+        out.append((m, om))
+    return out

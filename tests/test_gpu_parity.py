@@ -316,6 +316,50 @@ def test_solve_options(pkg, O):
     op.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_batch_uneven(precision, pkg, O):
+    """Independent problems of different sizes, ε and ω in one lockstep batched
+    solve: each field against the oracle CSC residual and the single solve."""
+    from maxwellfdfd import workloads as W
+    probs = W.uneven_problems(count=5, nmin=8, nmax=16, npml=2, nbox=4, seed=5)
+    ops, bs, As = [], [], []
+    for m, om in probs:
+        pkg.calc_matparams(m)
+        je, jm = pkg.create_srcs(m)
+        bs.append(pkg.create_b(pkg.EE, om, m, je, jm))
+        As.append(O.to_scipy(O.create_A(spec_of(m), om, m.εarr, m.μarr)))
+        ops.append(pkg.MaxwellOperator(m, om))
+    say("batch uneven", precision, [m.grid.N[0] for m, _ in probs])
+    es, st = pkg.solve_batch(ops, bs, tol=1e-8, precision=precision)
+    for p in range(len(ops)):
+        rel = np.linalg.norm(bs[p] - As[p] @ es[p]) / np.linalg.norm(bs[p])
+        say("  problem", p, "iterations", st[p].iterations, "fp32", st[p].iterations_fp32, "relres", rel)
+        assert st[p].converged and rel <= 1.05e-8
+        e1, _ = ops[p].solve(bs[p], tol=1e-8, precision="fp64")
+        assert np.linalg.norm(es[p] - e1) / np.linalg.norm(e1) <= 1e-6
+    with pytest.raises(pkg.MfdArgumentError):
+        pkg.solve_batch([ops[0], ops[0]], [bs[0], bs[0]])
+    with pytest.raises(pkg.MfdArgumentError):
+        pkg.solve_batch([ops[0], ops[1]], [bs[1], bs[0]])
+    for op in ops:
+        op.close()
+
+
+def test_bench_iteration_and_mixed_stats(pkg):
+    """Per-kernel timing entry point, and the FP32/FP64 iteration split of a mixed solve."""
+    m, om, b = c1_like(pkg, N=10, npml=2, r=2.0, lam=7.0)
+    op = pkg.MaxwellOperator(m, om)
+    for prec in ("fp64", "mixed"):
+        say("bench iteration", prec)
+        ms = op.bench_iteration(1, prec, 3)
+        assert len(ms) == 3 and all(t > 0 for t in ms)
+    e, st = op.solve(b, tol=1e-8, precision="mixed")
+    assert st.converged and 0 < st.iterations_fp32 <= st.iterations
+    e, st = op.solve(b, tol=1e-8, precision="fp64")
+    assert st.converged and st.iterations_fp32 == 0
+    op.close()
+
+
 def test_solve_iteration_cap_reports(pkg):
     m, om, b = c1_like(pkg, N=10, npml=2, r=2.0, lam=7.0)
     op = pkg.MaxwellOperator(m, om)
