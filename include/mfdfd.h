@@ -108,6 +108,10 @@ int mfd_distweights(double c, int gt, const double bounds[2], const double *l, c
  * jarr: (Nx,Ny,Nz,3) complex, accumulated into. */
 int mfd_add_pointsrc(const mfd_grid *g, int ft, const double c[3], const double p[3], mfd_c64 Idr,
                      mfd_c64 *jarr);
+/* add_src!(…, ::PlaneSrc) (planesrc.jl:41-79, via add_srce!/add_srcm!,
+ * model.jl:184-200): plane normal to axis nn (0..2) at intercept c, current
+ * J∆n along p (normalised here), added into jarr (Nx,Ny,Nz,3) column-major. */
+int mfd_add_planesrc(const mfd_grid *g, int ft, int nn, double c, const double p[3], mfd_c64 Jdn, mfd_c64 *jarr);
 
 /* create_curl (model.jl:171-172): which 0 = Cₑ (E -> H), 1 = Cₘ (H -> E).
  * Two phases: query nnz, then fill caller-allocated buffers. */
@@ -185,6 +189,12 @@ int mfd_solve_batch(int count, mfd_op *const *ops, const mfd_c64 *const *b, mfd_
 /* The same with device pointers on the context's GPU. */
 int mfd_solve_batch_dev(int count, mfd_op *const *ops, const void *const *b, void *const *e,
                         const mfd_solve_opts *o, mfd_solve_stats *st);
+/* h_from_e (model.jl:276-279): h = (i/ω) Pμ⁻¹ (Cₑ e + jₘ) with the operator's
+ * grid, μ and ω; e, jm, h host DOF vectors of 3M values (jm may be null). */
+int mfd_h_from_e(mfd_op *op, const mfd_c64 *e, const mfd_c64 *jm, mfd_c64 *h);
+/* e_from_h (model.jl:281-284): e = (−i/ω) Pε⁻¹ (Cₘ h − jₑ); diagonal ε only
+ * (MFD_ERR_UNSUPPORTED for an operator with off-diagonal ε); je may be null. */
+int mfd_e_from_h(mfd_op *op, const mfd_c64 *h, const mfd_c64 *je, mfd_c64 *e);
 int mfd_solve_adjoint(mfd_op *op, const mfd_c64 *g, mfd_c64 *lam, const mfd_solve_opts *o,
                       mfd_solve_stats *st);
 /* ∂f/∂ε per E DOF for real f with Aᵀλ = (∂f/∂e)ᵀ: grad = 2ω² Re(λ ∘ e)

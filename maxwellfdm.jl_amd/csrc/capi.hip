@@ -22,6 +22,7 @@ int solve_batch_impl(int count, mfd_op *const *ops, const mfd_c64 *const *b, mfd
                      const mfd_solve_opts *o, mfd_solve_stats *st);
 int solve_batch_dev_impl(int count, mfd_op *const *ops, const void *const *b, void *const *e,
                          const mfd_solve_opts *o, mfd_solve_stats *st);
+int field_convert_impl(mfd_op *op, int which, const mfd_c64 *in, const mfd_c64 *src, mfd_c64 *out);
 int solve_adjoint_impl(mfd_op *op, const mfd_c64 *g, mfd_c64 *lam, const mfd_solve_opts *o, mfd_solve_stats *st);
 int adjoint_grad_impl(mfd_op *op, const mfd_c64 *e, const mfd_c64 *lam, double *grad);
 int create_b_impl(mfd_ctx *ctx, const mfd::DevGrid &G, mfd_c64 omega, const mfd_c64 *mu_arr, const mfd_c64 *je,
@@ -147,6 +148,41 @@ int mfd_add_pointsrc(const mfd_grid *g, int ft, const double c[3], const double 
                     J.re += ir * wv;
                     J.im += ii * wv;
                 }
+    }
+    return MFD_OK;
+}
+
+// planesrc.jl:41-79 for K = Kf = 3: along the normal axis nn, distweights of
+// the intercept c pick two planes of J_w's grid type; J_w = J∆n p_w wt is added
+// over each whole plane
+int mfd_add_planesrc(const mfd_grid *g, int ft, int nn, double c, const double p[3], mfd_c64 Jdn, mfd_c64 *jarr) {
+    HGrid h;
+    MFD_TRY(build_hgrid(g, &h));
+    if (ft != MFD_EE && ft != MFD_HH) { set_error("ft must be EE or HH"); return MFD_ERR_ARG; }
+    if (nn < 0 || nn > 2) { set_error("plane normal must be a Cartesian axis 0..2"); return MFD_ERR_ARG; }
+    double nrm = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    if (!(nrm > 0)) { set_error("plane source polarisation must be nonzero"); return MFD_ERR_ARG; }
+    if (!jarr) { set_error("null current array"); return MFD_ERR_ARG; }
+    double pn[3] = {p[0] / nrm, p[1] / nrm, p[2] / nrm};
+    for (int w = 0; w < 3; ++w) {
+        int gt = gtype(h.bft, ft, w, nn);
+        double b2[2] = {h.bnd[0][nn], h.bnd[1][nn]};
+        int64_t ind[2];
+        double wt[2];
+        MFD_TRY(mfd_distweights(c, gt, b2, h.l[gt][nn].data(), h.dl[gt][nn].data(), h.N[nn], h.bloch[nn], ind, wt));
+        double jr = Jdn.re * pn[w], ji = Jdn.im * pn[w];
+        for (int k = 0; k < 2; ++k) {
+            int64_t lo[3] = {0, 0, 0}, hi[3] = {h.N[0] - 1, h.N[1] - 1, h.N[2] - 1};
+            lo[nn] = hi[nn] = ind[k] - 1;
+            double vr = jr * wt[k], vi = ji * wt[k];
+            for (int64_t z = lo[2]; z <= hi[2]; ++z)
+                for (int64_t y = lo[1]; y <= hi[1]; ++y)
+                    for (int64_t x = lo[0]; x <= hi[0]; ++x) {
+                        mfd_c64 &J = jarr[x + h.N[0] * (y + h.N[1] * z) + h.M * w];
+                        J.re += vr;
+                        J.im += vi;
+                    }
+        }
     }
     return MFD_OK;
 }
@@ -302,3 +338,14 @@ int mfd_adjoint_grad(mfd_op *op, const mfd_c64 *e, const mfd_c64 *lam, double *g
 }
 
 }  // extern "C"
+
+int mfd_h_from_e(mfd_op *op, const mfd_c64 *e, const mfd_c64 *jm, mfd_c64 *h) {
+    if (!op || !e || !h) { set_error("null argument"); return MFD_ERR_ARG; }
+    CHECK_CTX(op->ctx);
+    return field_convert_impl(op, 0, e, jm, h);
+}
+int mfd_e_from_h(mfd_op *op, const mfd_c64 *h, const mfd_c64 *je, mfd_c64 *e) {
+    if (!op || !h || !e) { set_error("null argument"); return MFD_ERR_ARG; }
+    CHECK_CTX(op->ctx);
+    return field_convert_impl(op, 1, h, je, e);
+}

@@ -198,6 +198,52 @@ __device__ inline bool reduce_last(double (&v)[NV], double *part, unsigned *cnt,
     return reduce_last_n<NV>(v, part, cnt, tot, (int)blockIdx.x, (int)gridDim.x);
 }
 
+// Grid reductions on the solver's hot path: every block writes its partial
+// sums with plain stores and a one-block finish kernel, launched next on the
+// same stream, adds them in a fixed order (deterministic) and updates the
+// solver scalars.  The kernel boundary orders the two, so no block needs a
+// device-scope fence or a shared counter.
+template <int NV>
+__device__ inline void block_partial(double (&v)[NV], double *part, int bid) {
+    block_sum<NV>(v);
+    if (threadIdx.x == 0)
+        for (int a = 0; a < NV; ++a) part[4 * bid + a] = v[a];
+}
+// fixed-order sum of nb partials by one block; result in thread 0
+template <int NV>
+__device__ inline void sum_partials(const double *part, int nb, double (&tot)[NV]) {
+    for (int a = 0; a < NV; ++a) tot[a] = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int a = 0; a < NV; ++a) tot[a] += part[4 * b + a];
+    block_sum<NV>(tot);
+}
+
+// after the stencil with EPI_DOT: σ = qᵀPq, α = ρ/σ (one block per RHS)
+__global__ __launch_bounds__(BS) void k_fin_dot(const double *part, int nb, Scal *scal) {
+    const int rhs = blockIdx.x;
+    Scal &S = scal[rhs];
+    if (S.done) return;
+    double tot[2];
+    sum_partials<2>(part + (size_t)rhs * NPART * 4, nb, tot);
+    if (threadIdx.x != 0) return;
+    if (S.conv) { S.done = 1; return; }
+    S.sig_re = tot[0];
+    S.sig_im = tot[1];
+    if (tot[0] == 0.0 && tot[1] == 0.0) { S.breakdown = 1; S.conv = 1; S.alpha_re = S.alpha_im = 0; return; }
+    c64 a = cdiv(mk<double>(S.rho_re, S.rho_im), mk<double>(tot[0], tot[1]));
+    S.alpha_re = a.re;
+    S.alpha_im = a.im;
+}
+
+// after the stencil with EPI_RESID: ||b − A x||² per (unmasked) RHS
+__global__ __launch_bounds__(BS) void k_fin_resid(const double *part, int nb, double *rn, const int *mask) {
+    const int rhs = blockIdx.x;
+    if (mask && !mask[rhs]) return;
+    double tot[1];
+    sum_partials<1>(part + (size_t)rhs * NPART * 4, nb, tot);
+    if (threadIdx.x == 0) rn[rhs] = tot[0];
+}
+
 // ------------------------------------------------------------ the stencil
 
 // y = A x (MODE_A) or y = P x (MODE_P) for the RHS blockIdx.y.
@@ -206,7 +252,7 @@ __device__ inline bool reduce_last(double (&v)[NV], double *part, unsigned *cnt,
 template <typename R, int MODE, bool OFFD, bool MU, int EPI>
 __global__ __launch_bounds__(BS) void k_apply(DGrid<R> g, OpData<R> od, const cx<R> *__restrict__ xall,
                                               cx<R> *__restrict__ yall, const c64 *__restrict__ ball, int64_t vs,
-                                              double *part, unsigned *cnt, Scal *scal, double *rn,
+                                              double *part, const Scal *scal,
                                               const int *__restrict__ mask = nullptr) {
     const int rhs = blockIdx.y;
     const cx<R> *x = xall + rhs * vs;
@@ -305,31 +351,19 @@ __global__ __launch_bounds__(BS) void k_apply(DGrid<R> g, OpData<R> od, const cx
         }
     }
     if (EPI == EPI_STORE) return;
-    const int slot_stride = NPART * 4;
-    double tot[2];
-    if (!reduce_last<2>(acc, part + (size_t)rhs * slot_stride, cnt + rhs, tot)) return;
-    if (threadIdx.x != 0) return;
-    if (EPI == EPI_RESID) { rn[rhs] = tot[0]; return; }
-    Scal &S = scal[rhs];
-    if (skip) { S.done = 1; return; }
-    S.sig_re = tot[0];
-    S.sig_im = tot[1];
-    if (tot[0] == 0.0 && tot[1] == 0.0) { S.breakdown = 1; S.conv = 1; S.alpha_re = S.alpha_im = 0; return; }
-    c64 a = cdiv(mk<double>(S.rho_re, S.rho_im), mk<double>(tot[0], tot[1]));
-    S.alpha_re = a.re;
-    S.alpha_im = a.im;
+    block_partial<2>(acc, part + (size_t)rhs * NPART * 4, (int)blockIdx.x);
 }
 
 #include "stencil_tiled.hpp"
 
 // ------------------------------------------------------- vector kernels
 
-// r -= α t; reduce ||r||² and rᵀr; SQMR scalar update in the last block
+// r -= α t; partial sums of ||r||² and rᵀr (k_fin_r finishes the SQMR step)
 template <typename R>
 __global__ __launch_bounds__(BS) void k_upd_r(cx<R> *__restrict__ rall, const cx<R> *__restrict__ tall, int64_t n,
-                                              int64_t vs, double *part, unsigned *cnt, Scal *scal) {
+                                              int64_t vs, double *part, const Scal *scal) {
     const int rhs = blockIdx.y;
-    Scal &S = scal[rhs];
+    const Scal &S = scal[rhs];
     if (S.done || S.conv) return;
     cx<R> *r = rall + rhs * vs;
     const cx<R> *t = tall + rhs * vs;
@@ -342,8 +376,16 @@ __global__ __launch_bounds__(BS) void k_upd_r(cx<R> *__restrict__ rall, const cx
         acc[1] += (double)v.re * v.re - (double)v.im * v.im;
         acc[2] += 2.0 * (double)v.re * v.im;
     }
+    block_partial<3>(acc, part + (size_t)rhs * NPART * 4, (int)blockIdx.x);
+}
+
+// after k_upd_r: ||r||, θ, c, τ, β, ρ of the SQMR step (one block per RHS)
+__global__ __launch_bounds__(BS) void k_fin_r(const double *part, int nb, Scal *scal) {
+    const int rhs = blockIdx.x;
+    Scal &S = scal[rhs];
+    if (S.done || S.conv) return;
     double tot[3];
-    if (!reduce_last<3>(acc, part + (size_t)rhs * NPART * 4, cnt + rhs, tot)) return;
+    sum_partials<3>(part + (size_t)rhs * NPART * 4, nb, tot);
     if (threadIdx.x != 0) return;
     double rn = sqrt(tot[0]);
     double th1 = rn / S.tau, c = 1.0 / sqrt(1.0 + th1 * th1);
@@ -710,9 +752,16 @@ template <typename R> inline const OpData<R> &opd(const mfd_op *op);
 template <> inline const OpData<double> &opd<double>(const mfd_op *op) { return op->d64; }
 template <> inline const OpData<float> &opd<float>(const mfd_op *op) { return op->d32; }
 
+// finish kernel of a stencil epilogue with a reduction (nb partials per RHS)
+template <int EPI>
+static void launch_finish(hipStream_t s, int nb, int64_t nrhs, double *part, Scal *scal, double *rn, const int *mask) {
+    if (EPI == EPI_DOT) k_fin_dot<<<(unsigned)nrhs, BS, 0, s>>>(part, nb, scal);
+    else if (EPI == EPI_RESID) k_fin_resid<<<(unsigned)nrhs, BS, 0, s>>>(part, nb, rn, mask);
+}
+
 // launch one operator apply
 template <typename R, int MODE, int EPI>
-static void launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int64_t nrhs, double *part, unsigned *cnt,
+static void launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int64_t nrhs, double *part,
                          Scal *scal, double *rn, const int *mask = nullptr) {
     const DGrid<R> &g = op->G.view<R>();
     const OpData<R> &od = opd<R>(op);
@@ -726,24 +775,26 @@ static void launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int
         while ((int64_t)nx * ny * nch > NPART) { kz *= 2; nch = (g.N[2] + kz - 1) / kz; }
         dim3 grid(nx, ny, (unsigned)(nch * nrhs));
         if (MODE == MODE_P && op->gd) {
-            if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, true><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, cnt, scal, rn, mask);
-            else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, true><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, cnt, scal, rn, mask);
-            else k_apply_tiled<R, MODE, EPI, 1, true><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, cnt, scal, rn, mask);
+            if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, true><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask);
+            else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, true><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask);
+            else k_apply_tiled<R, MODE, EPI, 1, true><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask);
         } else {
-            if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, false><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, cnt, scal, rn, mask);
-            else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, false><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, cnt, scal, rn, mask);
-            else k_apply_tiled<R, MODE, EPI, 1, false><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, cnt, scal, rn, mask);
+            if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, false><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask);
+            else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, false><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask);
+            else k_apply_tiled<R, MODE, EPI, 1, false><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask);
         }
+        launch_finish<EPI>(s, nx * ny * nch, nrhs, part, scal, rn, mask);
         return;
     }
     dim3 grid(ngrid(g.M), (unsigned)nrhs);
     if (op->full) {
-        if (op->has_mu) k_apply<R, MODE, true, true, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, cnt, scal, rn, mask);
-        else k_apply<R, MODE, true, false, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, cnt, scal, rn, mask);
+        if (op->has_mu) k_apply<R, MODE, true, true, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, scal, mask);
+        else k_apply<R, MODE, true, false, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, scal, mask);
     } else {
-        if (op->has_mu) k_apply<R, MODE, false, true, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, cnt, scal, rn, mask);
-        else k_apply<R, MODE, false, false, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, cnt, scal, rn, mask);
+        if (op->has_mu) k_apply<R, MODE, false, true, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, scal, mask);
+        else k_apply<R, MODE, false, false, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, scal, mask);
     }
+    launch_finish<EPI>(s, (int)grid.x, nrhs, part, scal, rn, mask);
 }
 
 template <typename R>
@@ -753,8 +804,9 @@ static void launch_iter(mfd_op *op, int64_t nrhs) {
     hipStream_t s = op->stream;
     dim3 gv(ngrid(n), (unsigned)nrhs);
     size_t pslot = (size_t)nrhs * NPART * 4;
-    launch_apply<R, MODE_P, EPI_DOT>(op, (const cx<R> *)w.q, (cx<R> *)w.t, nullptr, nrhs, w.part, w.cnt, w.scal, nullptr);
-    k_upd_r<R><<<gv, BS, 0, s>>>((cx<R> *)w.r, (const cx<R> *)w.t, n, n, w.part + pslot, w.cnt + nrhs, w.scal);
+    launch_apply<R, MODE_P, EPI_DOT>(op, (const cx<R> *)w.q, (cx<R> *)w.t, nullptr, nrhs, w.part, w.scal, nullptr);
+    k_upd_r<R><<<gv, BS, 0, s>>>((cx<R> *)w.r, (const cx<R> *)w.t, n, n, w.part + pslot, w.scal);
+    k_fin_r<<<(unsigned)nrhs, BS, 0, s>>>(w.part + pslot, (int)gv.x, w.scal);
     k_upd_dzq<R><<<gv, BS, 0, s>>>((cx<R> *)w.d, (cx<R> *)w.z, (cx<R> *)w.q, (const cx<R> *)w.r, n, n, w.scal);
 }
 
@@ -883,8 +935,8 @@ int op_set_graddiv_impl(mfd_op *op, int enable) {
 
 int op_apply_dev_impl(mfd_op *op, const void *x, void *y, int64_t nrhs, int prec) {
     if (nrhs < 1 || nrhs > 65535) { set_error("nrhs out of range"); return MFD_ERR_ARG; }
-    if (prec == MFD_FP64) launch_apply<double, MODE_A, EPI_STORE>(op, (const c64 *)x, (c64 *)y, nullptr, nrhs, nullptr, nullptr, nullptr, nullptr);
-    else launch_apply<float, MODE_A, EPI_STORE>(op, (const c32 *)x, (c32 *)y, nullptr, nrhs, nullptr, nullptr, nullptr, nullptr);
+    if (prec == MFD_FP64) launch_apply<double, MODE_A, EPI_STORE>(op, (const c64 *)x, (c64 *)y, nullptr, nrhs, nullptr, nullptr, nullptr);
+    else launch_apply<float, MODE_A, EPI_STORE>(op, (const c32 *)x, (c32 *)y, nullptr, nrhs, nullptr, nullptr, nullptr);
     MFD_HIP(hipGetLastError());
     MFD_HIP(hipStreamSynchronize(op->stream));
     return MFD_OK;
@@ -907,7 +959,8 @@ int op_apply_impl(mfd_op *op, const mfd_c64 *x, mfd_c64 *y, int64_t nrhs) {
     return rc;
 }
 
-int op_bench_apply_impl(mfd_op *op, int64_t nrhs, int prec, int iters, double *ms) {
+// average ms of one stencil launch (EPI_STORE): MODE_A or MODE_P, FP64 or FP32 vectors
+static int time_apply(mfd_op *op, int64_t nrhs, int prec, int mode, int iters, double *ms) {
     int64_t n = 3 * op->G.h.M;
     size_t es = prec == MFD_FP64 ? sizeof(c64) : sizeof(c32);
     void *dx = nullptr, *dy = nullptr;
@@ -916,8 +969,13 @@ int op_bench_apply_impl(mfd_op *op, int64_t nrhs, int prec, int iters, double *m
     hipStream_t s = op->stream;
     hipMemsetAsync(dx, 0, es * n * nrhs, s);
     auto one = [&]() {
-        if (prec == MFD_FP64) launch_apply<double, MODE_A, EPI_STORE>(op, (const c64 *)dx, (c64 *)dy, nullptr, nrhs, nullptr, nullptr, nullptr, nullptr);
-        else launch_apply<float, MODE_P, EPI_STORE>(op, (const c32 *)dx, (c32 *)dy, nullptr, nrhs, nullptr, nullptr, nullptr, nullptr);
+        if (prec == MFD_FP64) {
+            if (mode == MODE_A) launch_apply<double, MODE_A, EPI_STORE>(op, (const c64 *)dx, (c64 *)dy, nullptr, nrhs, nullptr, nullptr, nullptr);
+            else launch_apply<double, MODE_P, EPI_STORE>(op, (const c64 *)dx, (c64 *)dy, nullptr, nrhs, nullptr, nullptr, nullptr);
+        } else {
+            if (mode == MODE_A) launch_apply<float, MODE_A, EPI_STORE>(op, (const c32 *)dx, (c32 *)dy, nullptr, nrhs, nullptr, nullptr, nullptr);
+            else launch_apply<float, MODE_P, EPI_STORE>(op, (const c32 *)dx, (c32 *)dy, nullptr, nrhs, nullptr, nullptr, nullptr);
+        }
     };
     for (int w = 0; w < 3; ++w) one();
     hipEventRecord(op->ev0, s);
@@ -931,6 +989,10 @@ int op_bench_apply_impl(mfd_op *op, int64_t nrhs, int prec, int iters, double *m
     hipFree(dy);
     if (e != hipSuccess) { set_error(hipGetErrorString(e)); return MFD_ERR_HIP; }
     return MFD_OK;
+}
+
+int op_bench_apply_impl(mfd_op *op, int64_t nrhs, int prec, int iters, double *ms) {
+    return time_apply(op, nrhs, prec, prec == MFD_FP64 ? MODE_A : MODE_P, iters, ms);
 }
 
 // Scalars of an active iteration with α = β = 0, so that repeated launches of
@@ -965,12 +1027,13 @@ int op_bench_iteration_impl(mfd_op *op, int64_t nrhs, int prec, int iters, doubl
         auto one = [&]() {
             if (which == 0) {
                 if (prec == MFD_FP64)
-                    launch_apply<double, MODE_P, EPI_DOT>(op, (const c64 *)w.q, (c64 *)w.t, nullptr, nrhs, w.part, w.cnt, w.scal, nullptr);
-                else launch_apply<float, MODE_P, EPI_DOT>(op, (const c32 *)w.q, (c32 *)w.t, nullptr, nrhs, w.part, w.cnt, w.scal, nullptr);
+                    launch_apply<double, MODE_P, EPI_DOT>(op, (const c64 *)w.q, (c64 *)w.t, nullptr, nrhs, w.part, w.scal, nullptr);
+                else launch_apply<float, MODE_P, EPI_DOT>(op, (const c32 *)w.q, (c32 *)w.t, nullptr, nrhs, w.part, w.scal, nullptr);
             } else if (which == 1) {
                 size_t ps = (size_t)nrhs * NPART * 4;
-                if (prec == MFD_FP64) k_upd_r<double><<<gv, BS, 0, s>>>((c64 *)w.r, (const c64 *)w.t, n, n, w.part + ps, w.cnt + nrhs, w.scal);
-                else k_upd_r<float><<<gv, BS, 0, s>>>((c32 *)w.r, (const c32 *)w.t, n, n, w.part + ps, w.cnt + nrhs, w.scal);
+                if (prec == MFD_FP64) k_upd_r<double><<<gv, BS, 0, s>>>((c64 *)w.r, (const c64 *)w.t, n, n, w.part + ps, w.scal);
+                else k_upd_r<float><<<gv, BS, 0, s>>>((c32 *)w.r, (const c32 *)w.t, n, n, w.part + ps, w.scal);
+                k_fin_r<<<(unsigned)nrhs, BS, 0, s>>>(w.part + ps, (int)gv.x, w.scal);
             } else {
                 if (prec == MFD_FP64) k_upd_dzq<double><<<gv, BS, 0, s>>>((c64 *)w.d, (c64 *)w.z, (c64 *)w.q, (const c64 *)w.r, n, n, w.scal);
                 else k_upd_dzq<float><<<gv, BS, 0, s>>>((c32 *)w.d, (c32 *)w.z, (c32 *)w.q, (const c32 *)w.r, n, n, w.scal);
@@ -995,8 +1058,8 @@ int op_bench_iteration_impl(mfd_op *op, int64_t nrhs, int prec, int iters, doubl
     return MFD_OK;
 }
 
-// Tile-shape autotuner: times the solver's stencils (FP64 A with the residual
-// epilogue shape, complex-FP32 inner operator) for each candidate (TY, KZ) and
+// Tile-shape autotuner: times the Krylov iteration's stencil (the Jacobi-scaled
+// operator P on FP64 or complex-FP32 vectors) for each candidate (TY, KZ) and
 // keeps the fastest per precision.
 int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best32, double *ms_out) {
     if (op->full || op->has_mu) {   // the direct kernel serves these operators
@@ -1015,7 +1078,7 @@ int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best
                 tile[0] = tys[a];
                 tile[1] = kzs[b];
                 double ms = 0;
-                MFD_TRY(op_bench_apply_impl(op, nrhs, prec == 0 ? MFD_FP64 : MFD_MIXED, iters, &ms));
+                MFD_TRY(time_apply(op, nrhs, prec == 0 ? MFD_FP64 : MFD_MIXED, MODE_P, iters, &ms));
                 if (ms < bestms[prec]) { bestms[prec] = ms; bt[prec][0] = tys[a]; bt[prec][1] = kzs[b]; }
             }
     op->tile64[0] = bt[0][0]; op->tile64[1] = bt[0][1];
@@ -1039,13 +1102,13 @@ int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best
 // the search direction p is kept.  Convergence is decided on that FP64
 // residual only.
 
-// x += α p;  r −= α t;  reduce ||r||², rᵀr;  β, ρ, running max in the last block
+// x += α p;  r −= α t;  partial sums of ||r||², rᵀr (k_fin_cocg finishes)
 template <typename R>
 __global__ __launch_bounds__(BS) void k_cocg_xr(cx<R> *__restrict__ xall, cx<R> *__restrict__ rall,
                                                 const cx<R> *__restrict__ pall, const cx<R> *__restrict__ tall,
-                                                int64_t n, int64_t vs, double *part, unsigned *cnt, Scal *scal) {
+                                                int64_t n, int64_t vs, double *part, const Scal *scal) {
     const int rhs = blockIdx.y;
-    Scal &S = scal[rhs];
+    const Scal &S = scal[rhs];
     if (S.done || S.breakdown) return;
     cx<R> *x = xall + rhs * vs, *r = rall + rhs * vs;
     const cx<R> *p = pall + rhs * vs, *t = tall + rhs * vs;
@@ -1059,8 +1122,16 @@ __global__ __launch_bounds__(BS) void k_cocg_xr(cx<R> *__restrict__ xall, cx<R> 
         acc[1] += (double)v.re * v.re - (double)v.im * v.im;
         acc[2] += 2.0 * (double)v.re * v.im;
     }
+    block_partial<3>(acc, part + (size_t)rhs * NPART * 4, (int)blockIdx.x);
+}
+
+// after k_cocg_xr: β, ρ, running max of ||r|| (one block per RHS)
+__global__ __launch_bounds__(BS) void k_fin_cocg(const double *part, int nb, Scal *scal) {
+    const int rhs = blockIdx.x;
+    Scal &S = scal[rhs];
+    if (S.done || S.breakdown) return;
     double tot[3];
-    if (!reduce_last<3>(acc, part + (size_t)rhs * NPART * 4, cnt + rhs, tot)) return;
+    sum_partials<3>(part + (size_t)rhs * NPART * 4, nb, tot);
     if (threadIdx.x != 0) return;
     c64 rho1 = mk<double>(tot[1], tot[2]);
     if (S.rho_re == 0.0 && S.rho_im == 0.0) { S.breakdown = 1; S.beta_re = S.beta_im = 0; }
@@ -1173,9 +1244,10 @@ static void launch_cocg_iter(mfd_op *op, int64_t nrhs) {
     dim3 gv(ngrid(n), (unsigned)nrhs);
     size_t pslot = (size_t)nrhs * NPART * 4;
     // w.q holds p, w.t holds t = P p, w.z holds x, w.r holds r
-    launch_apply<R, MODE_P, EPI_DOT>(op, (const cx<R> *)w.q, (cx<R> *)w.t, nullptr, nrhs, w.part, w.cnt, w.scal, nullptr);
+    launch_apply<R, MODE_P, EPI_DOT>(op, (const cx<R> *)w.q, (cx<R> *)w.t, nullptr, nrhs, w.part, w.scal, nullptr);
     k_cocg_xr<R><<<gv, BS, 0, s>>>((cx<R> *)w.z, (cx<R> *)w.r, (const cx<R> *)w.q, (const cx<R> *)w.t, n, n,
-                                   w.part + pslot, w.cnt + nrhs, w.scal);
+                                   w.part + pslot, w.scal);
+    k_fin_cocg<<<(unsigned)nrhs, BS, 0, s>>>(w.part + pslot, (int)gv.x, w.scal);
     k_cocg_p<R><<<gv, BS, 0, s>>>((cx<R> *)w.q, (const cx<R> *)w.r, n, n, w.scal);
 }
 
@@ -1207,7 +1279,7 @@ static int run_cocg(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_
     auto residual = [&](double *worst) -> int {   // r64 = b − A e for masked RHS; marks converged ones
         MFD_HIP(hipMemcpyAsync(mask_d, mask.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s));
         launch_apply<double, MODE_A, EPI_RESID>(op, (const c64 *)e, (c64 *)w.r64, (const c64 *)b, nrhs,
-                                                w.part + 2 * pslot, w.cnt + 2 * nrhs, nullptr, w.rn, mask_d);
+                                                w.part + 2 * pslot, nullptr, w.rn, mask_d);
         MFD_HIP(hipGetLastError());
         MFD_HIP(hipMemcpyAsync(w.rn_h, w.rn, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
         MFD_HIP(hipStreamSynchronize(s));
@@ -1388,7 +1460,7 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
             k_probe_x<R><<<gv2, BS, 0, s>>>(op->G.d64, op->d64, (const c64 *)e, (const cx<R> *)w.z, (c64 *)w.et, n, n);
             size_t ps = (size_t)nrhs * NPART * 4;
             launch_apply<double, MODE_A, EPI_RESID>(op, (const c64 *)w.et, (c64 *)w.et2, (const c64 *)b, nrhs,
-                                                    w.part + 2 * ps, w.cnt + 2 * nrhs, nullptr, w.rn);
+                                                    w.part + 2 * ps, nullptr, w.rn);
             MFD_HIP(hipMemcpyAsync(w.rn_h, w.rn, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
             MFD_HIP(hipStreamSynchronize(s));
             double worst = 0;
@@ -1455,7 +1527,7 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
     int cyc_prec = prec;
     while (true) {
         launch_apply<double, MODE_A, EPI_RESID>(op, (const c64 *)e, (c64 *)w.r64, (const c64 *)b, nrhs,
-                                                w.part + 2 * pslot, w.cnt + 2 * nrhs, nullptr, w.rn);
+                                                w.part + 2 * pslot, nullptr, w.rn);
         MFD_HIP(hipGetLastError());
         MFD_HIP(hipMemcpyAsync(w.rn_h, w.rn, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
         MFD_HIP(hipStreamSynchronize(s));
@@ -1614,7 +1686,7 @@ int solve_batch_dev_impl(int count, mfd_op *const *ops, const void *const *b, vo
             int64_t n = w.n;
             dim3 gv(ngrid(n), 1);
             launch_apply<double, MODE_A, EPI_RESID>(op, (const c64 *)B.e, (c64 *)w.r64, (const c64 *)B.b, 1,
-                                                    w.part + 2 * NPART * 4, w.cnt + 2, nullptr, w.rn);
+                                                    w.part + 2 * NPART * 4, nullptr, w.rn);
             MFD_HIP(hipMemcpyAsync(w.rn_h, w.rn, sizeof(double), hipMemcpyDeviceToHost, s));
             MFD_HIP(hipStreamSynchronize(s));
             double rel = B.bn > 0 ? sqrt(w.rn_h[0] / B.bn) : 0.0;
@@ -1720,7 +1792,7 @@ int solve_batch_dev_impl(int count, mfd_op *const *ops, const void *const *b, vo
                     dim3 gv(ngrid(w.n), 1);
                     k_probe_x<float><<<gv, BS, 0, s>>>(B.op->G.d64, B.op->d64, (const c64 *)B.e, (const c32 *)w.z, (c64 *)w.et, w.n, w.n);
                     launch_apply<double, MODE_A, EPI_RESID>(B.op, (const c64 *)w.et, (c64 *)w.et2, (const c64 *)B.b, 1,
-                                                            w.part + 2 * NPART * 4, w.cnt + 2, nullptr, w.rn);
+                                                            w.part + 2 * NPART * 4, nullptr, w.rn);
                     BCHK(hipMemcpyAsync(w.rn_h, w.rn, sizeof(double), hipMemcpyDeviceToHost, s));
                     BCHK(hipStreamSynchronize(s));
                     double pr = B.bn > 0 ? sqrt(w.rn_h[0] / B.bn) : 0.0;
@@ -1844,6 +1916,99 @@ int solve_adjoint_impl(mfd_op *op, const mfd_c64 *gvec, mfd_c64 *lam, const mfd_
     }
     hipFree(db);
     hipFree(de);
+    return rc;
+}
+
+// ------------------------------------------------ field post-processing
+
+// h = (i/ω) μ⁻¹ (Cₑ e + jₘ) at every H DOF (model.jl:278-279); jm may be null
+template <bool MU>
+__global__ __launch_bounds__(BS) void k_h_from_e(DGrid<double> g, OpData<double> od, const c64 *__restrict__ e,
+                                                 const c64 *__restrict__ jm, c64 *__restrict__ h, c64 iw) {
+    for (int64_t p = blockIdx.x * (int64_t)BS + threadIdx.x; p < g.M; p += (int64_t)gridDim.x * BS) {
+        int i, j, k;
+        cell3(g, p, i, j, k);
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            int64_t d = dofidx(g, u, p);
+            c64 v = Hval<double, MODE_A, false>(g, od, e, u, i, j, k);
+            if (jm) v = v + jm[d];
+            if (MU) v = v * od.muinv[3 * p + u];
+            h[d] = iw * v;
+        }
+    }
+}
+
+// H_c of a given H vector at (i,j,k) with one index possibly −1: the ghost
+// below the domain is the Bloch image (times e^{+ikL}⁻¹) or zero
+__device__ inline c64 ldH(const DGrid<double> &g, const c64 *__restrict__ h, int c, int i, int j, int k) {
+    c64 ph = cone<double>();
+    int id[3] = {i, j, k};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (id[a] < 0) {
+            if (!g.bloch[a]) return czero<double>();
+            id[a] += g.N[a];
+            ph = ph * g.phinv[a];
+        }
+    int64_t p = id[0] + (int64_t)g.N[0] * (id[1] + (int64_t)g.N[1] * id[2]);
+    return h[dofidx(g, c, p)] * ph;
+}
+
+// e = (−i/ω) ε⁻¹ (Cₘ h − jₑ) at every E DOF, diagonal ε (model.jl:283-284); je may be null
+__global__ __launch_bounds__(BS) void k_e_from_h(DGrid<double> g, OpData<double> od, const c64 *__restrict__ h,
+                                                 const c64 *__restrict__ je, c64 *__restrict__ e, c64 miw) {
+    for (int64_t p = blockIdx.x * (int64_t)BS + threadIdx.x; p < g.M; p += (int64_t)gridDim.x * BS) {
+        int i, j, k;
+        cell3(g, p, i, j, k);
+        int id[3] = {i, j, k};
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            int v = u == 0 ? 1 : (u == 1 ? 2 : 0), w = u == 0 ? 2 : (u == 1 ? 0 : 1);
+            int iv[3] = {i, j, k}, iw_[3] = {i, j, k};
+            iv[v] -= 1;
+            iw_[w] -= 1;
+            // (Cₘ h)_u = ∂_v h_w − ∂_w h_v (backward differences)
+            c64 cc = g.sdlinv[PRIM][v][id[v]] * (ldH(g, h, w, i, j, k) - ldH(g, h, w, iv[0], iv[1], iv[2])) -
+                     g.sdlinv[PRIM][w][id[w]] * (ldH(g, h, v, i, j, k) - ldH(g, h, v, iw_[0], iw_[1], iw_[2]));
+            int64_t d = dofidx(g, u, p);
+            if (je) cc = cc - je[d];
+            e[d] = miw * cdiv(cc, od.epsd[3 * p + u]);
+        }
+    }
+}
+
+// which = 0: out = h_from_e(in, src = jₘ); which = 1: out = e_from_h(in, src = jₑ)
+int field_convert_impl(mfd_op *op, int which, const mfd_c64 *in, const mfd_c64 *src, mfd_c64 *out) {
+    if (which == 1 && op->full) { set_error("e_from_h: only diagonal ε is supported (Pε \\ is a sparse solve otherwise)"); return MFD_ERR_UNSUPPORTED; }
+    if (op->omega.re == 0.0 && op->omega.im == 0.0) { set_error("h_from_e/e_from_h need ω ≠ 0"); return MFD_ERR_ARG; }
+    int64_t n = 3 * op->G.h.M;
+    size_t bytes = sizeof(c64) * n;
+    hipStream_t s = op->stream;
+    void *din = nullptr, *dsrc = nullptr, *dout = nullptr;
+    MFD_HIP(hipMalloc(&din, bytes));
+    int rc = MFD_OK;
+    if (hipMalloc(&dout, bytes) != hipSuccess || (src && hipMalloc(&dsrc, bytes) != hipSuccess)) { set_error("hipMalloc failed"); rc = MFD_ERR_ALLOC; }
+    if (!rc && (hipMemcpyAsync(din, in, bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+                (src && hipMemcpyAsync(dsrc, src, bytes, hipMemcpyHostToDevice, s) != hipSuccess)))
+        rc = MFD_ERR_HIP;
+    if (!rc) {
+        c64 iw = cdiv(mk<double>(0.0, 1.0), op->omega);   // i/ω
+        unsigned gr = ngrid(op->G.h.M);
+        if (which == 0) {
+            if (op->has_mu) k_h_from_e<true><<<gr, BS, 0, s>>>(op->G.d64, op->d64, (const c64 *)din, (const c64 *)dsrc, (c64 *)dout, iw);
+            else k_h_from_e<false><<<gr, BS, 0, s>>>(op->G.d64, op->d64, (const c64 *)din, (const c64 *)dsrc, (c64 *)dout, iw);
+        } else {
+            k_e_from_h<<<gr, BS, 0, s>>>(op->G.d64, op->d64, (const c64 *)din, (const c64 *)dsrc, (c64 *)dout,
+                                         mk<double>(-iw.re, -iw.im));
+        }
+        if (hipGetLastError() != hipSuccess || hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = MFD_ERR_HIP;
+    }
+    hipFree(din);
+    if (dout) hipFree(dout);
+    if (dsrc) hipFree(dsrc);
     return rc;
 }
 

@@ -107,8 +107,8 @@ template <typename R, int MODE, int EPI, int TY, bool GD>
 __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData<double> od, c64 om2,
                                                          const cx<R> *__restrict__ xall,
                                                          cx<R> *__restrict__ yall, const c64 *__restrict__ ball,
-                                                         int64_t vs, int KZ, int nch, double *part, unsigned *cnt,
-                                                         Scal *scal, double *rn, const int *__restrict__ mask = nullptr) {
+                                                         int64_t vs, int KZ, int nch, double *part,
+                                                         const Scal *scal, const int *__restrict__ mask = nullptr) {
     using C = TileCfg<TY>;
     constexpr int GB = GD ? 2 : 1, GH = GD ? C::XH : 1, GW = GD ? C::XW : 1;
     __shared__ c64 Xs[2][3][C::XH][C::XW];
@@ -368,18 +368,6 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
         }
     }
     if (EPI == EPI_STORE) return;
-    const int nb = gridDim.x * gridDim.y * nch;
     const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * chunk);
-    double tot[2];
-    if (!reduce_last_n<2>(acc, part + (size_t)rhs * NPART * 4, cnt + rhs, tot, bid, nb)) return;
-    if (threadIdx.x != 0) return;
-    if (EPI == EPI_RESID) { rn[rhs] = tot[0]; return; }
-    Scal &S = scal[rhs];
-    if (skip) { S.done = 1; return; }
-    S.sig_re = tot[0];
-    S.sig_im = tot[1];
-    if (tot[0] == 0.0 && tot[1] == 0.0) { S.breakdown = 1; S.conv = 1; S.alpha_re = S.alpha_im = 0; return; }
-    c64 a = cdiv(mk<double>(S.rho_re, S.rho_im), mk<double>(tot[0], tot[1]));
-    S.alpha_re = a.re;
-    S.alpha_im = a.im;
+    block_partial<2>(acc, part + (size_t)rhs * NPART * 4, bid);
 }

@@ -76,6 +76,8 @@ SIGNATURES = {
                                   _I64P, C.POINTER(C.c_double)]),
     "mfd_add_pointsrc": (C.c_int, [C.POINTER(mfd_grid), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                    c64, _P]),
+    "mfd_add_planesrc": (C.c_int, [C.POINTER(mfd_grid), C.c_int, C.c_int, C.c_double, C.POINTER(C.c_double),
+                                   c64, _P]),
     "mfd_curl_nnz": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int, _I64P]),
     "mfd_curl_csc": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int, _P, _P, _P]),
     "mfd_matparams": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int64, _P, _P, _P, _P, _P]),
@@ -99,6 +101,8 @@ SIGNATURES = {
     "mfd_solve_dev": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(mfd_solve_opts), C.POINTER(mfd_solve_stats)]),
     "mfd_solve_batch": (C.c_int, [C.c_int, _P, _P, _P, C.POINTER(mfd_solve_opts), _P]),
     "mfd_solve_batch_dev": (C.c_int, [C.c_int, _P, _P, _P, C.POINTER(mfd_solve_opts), _P]),
+    "mfd_h_from_e": (C.c_int, [_P, _P, _P, _P]),
+    "mfd_e_from_h": (C.c_int, [_P, _P, _P, _P]),
     "mfd_solve_adjoint": (C.c_int, [_P, _P, _P, C.POINTER(mfd_solve_opts), C.POINTER(mfd_solve_stats)]),
     "mfd_adjoint_grad": (C.c_int, [_P, _P, _P, _P]),
 }

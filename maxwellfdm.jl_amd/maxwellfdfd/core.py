@@ -15,7 +15,7 @@ from typing import List, Tuple
 import numpy as np
 
 from . import _lib as L
-from ._lib import check, context, lib, ptr
+from ._lib import MfdArgumentError, check, context, lib, ptr
 
 EE, HH = 0, 1            # FieldType
 PRIM, DUAL = 0, 1        # GridType
@@ -117,6 +117,22 @@ class PointSrc:
         p = np.asarray(p, dtype=np.float64)
         self.p = p / np.sqrt(np.sum(p * p))
         self.Idr = complex(Idr)
+
+
+class PlaneSrc:
+    """Plane source PlaneSrc(n, c, p, J∆n) (planesrc.jl:14-33): n is a Cartesian
+    normal, c the intercept along it, p the polarisation (normalised), J∆n the
+    complex sheet current."""
+
+    def __init__(self, n, c, p, Jdn=1.0):
+        n = np.asarray(n, dtype=np.float64)
+        if np.count_nonzero(n) != 1:
+            raise MfdArgumentError(L.MFD_ERR_ARG, f"n = {list(n)} must be along Cartesian direction.")
+        self.n = n / np.sqrt(np.sum(n * n))
+        self.c = float(c)
+        p = np.asarray(p, dtype=np.float64)
+        self.p = p / np.sqrt(np.sum(p * p))
+        self.Jdn = complex(Jdn)
 
 
 def distweights(c, gt, bounds, l, dl, isbloch):
@@ -236,9 +252,17 @@ def add_obj(mdl, matname, *shapes, ε=1.0, μ=1.0):
 
 def _add_src(mdl, ft, src):
     jarr = mdl.jₑarr if ft == EE else mdl.jₘarr
-    c = (C.c_double * 3)(*src.c)
     p = (C.c_double * 3)(*src.p)
     g = mdl._desc()
+    if isinstance(src, PlaneSrc):
+        # the reference locates the normal with findfirst(n .== 1) (planesrc.jl:42),
+        # so only a positive Cartesian normal names an axis
+        if not np.any(src.n == 1):
+            raise MfdArgumentError(L.MFD_ERR_ARG, f"plane normal {list(src.n)} must be a positive Cartesian direction")
+        nn = int(np.flatnonzero(src.n == 1)[0])
+        check(lib().mfd_add_planesrc(C.byref(g), ft, nn, src.c, p, L.cplx(src.Jdn), ptr(jarr)))
+        return
+    c = (C.c_double * 3)(*src.c)
     check(lib().mfd_add_pointsrc(C.byref(g), ft, c, p, L.cplx(src.Idr), ptr(jarr)))
 
 
@@ -392,7 +416,7 @@ def create_linsys(ft, ω, mdl, jₑ=None, jₘ=None):
 def _solve_batch(ops, bs, es, dev, o, raise_on_noconv=True):
     count = len(ops)
     if not (count == len(bs) == len(es)):
-        raise MfdArgumentError("solve_batch: one b and one e per operator")
+        raise MfdArgumentError(L.MFD_ERR_ARG, "solve_batch: one b and one e per operator")
     hs = (C.c_void_p * count)(*[op._h for op in ops])
     if dev:
         bp = (C.c_void_p * count)(*bs)
@@ -415,7 +439,7 @@ def solve_batch(ops, bs, tol=1e-8, precision="fp64", maxit=None, inner_tol=None,
     bs = [L.carr(b) for b in bs]
     for op, b in zip(ops, bs):
         if b.size != op.n:
-            raise MfdArgumentError("solve_batch: b has the wrong length for its operator")
+            raise MfdArgumentError(L.MFD_ERR_ARG, "solve_batch: b has the wrong length for its operator")
     es = [np.zeros_like(b) for b in bs]
     o = MaxwellOperator._opts(tol, precision, maxit, inner_tol, check_every, use_graph)
     st = _solve_batch(list(ops), bs, es, False, o, raise_on_noconv)
@@ -542,6 +566,22 @@ class MaxwellOperator:
         st = L.mfd_solve_stats()
         check(lib().mfd_solve_adjoint(self._h, ptr(g), ptr(lam), C.byref(o), C.byref(st)))
         return lam, st
+
+    def h_from_e(self, e, jm=None):
+        """h_from_e (model.jl:276-279): h = (i/ω) Pμ⁻¹ (Cₑ e + jₘ), on the device."""
+        e = L.carr(e)
+        h = np.zeros_like(e)
+        jm = None if jm is None else L.carr(jm)
+        check(lib().mfd_h_from_e(self._h, ptr(e), None if jm is None else ptr(jm), ptr(h)))
+        return h
+
+    def e_from_h(self, h, je=None):
+        """e_from_h (model.jl:281-284): e = (−i/ω) Pε⁻¹ (Cₘ h − jₑ), on the device (diagonal ε)."""
+        h = L.carr(h)
+        e = np.zeros_like(h)
+        je = None if je is None else L.carr(je)
+        check(lib().mfd_e_from_h(self._h, ptr(h), None if je is None else ptr(je), ptr(e)))
+        return e
 
     def adjoint_grad(self, e, lam):
         """∂f/∂ε per E DOF: 2 Re(ω² λ ∘ e) for Aᵀλ = (∂f/∂e)ᵀ."""

@@ -248,6 +248,33 @@ int og_add_pointsrc(const og_desc *d, int ft, const double c[3], const double p[
     return 0;
 }
 
+/* planesrc.jl:41-79 for K = Kf = 3 (field and shape spaces coincide, so
+ * J_w sits at gt_w(w, gt0)): along the normal axis nn, distweights of the
+ * intercept c give two planes and weights; J_w = J∆n p_w wt is added over the
+ * whole of each plane (planesrc.jl:71-77). */
+int og_add_planesrc(const og_desc *d, int ft, int nn, double c, const double p[3],
+                    double Jdn_re, double Jdn_im, oc64 *jarr) {
+    G g; int rc = g_build(d, &g); if (rc) return rc;
+    for (int nw = 0; nw < 3; ++nw) {
+        int gt = gtype(&g, ft, nw, nn);
+        double b2[2] = {g.bnd[0][nn], g.bnd[1][nn]};
+        int64_t ind[2]; double wt[2];
+        rc = og_distweights(c, gt, b2, g.l[gt][nn], g.dl[gt][nn], g.N[nn], g.bloch[nn], ind, wt);
+        if (rc) { g_free(&g); return rc; }
+        double jr = Jdn_re * p[nw], ji = Jdn_im * p[nw];
+        for (int k = 0; k < 2; ++k) {
+            oc64 v = jr * wt[k] + (ji * wt[k]) * I;
+            int64_t lo[3] = {0, 0, 0}, hi[3] = {g.N[0] - 1, g.N[1] - 1, g.N[2] - 1};
+            lo[nn] = hi[nn] = ind[k];   /* 0-based */
+            for (int64_t z = lo[2]; z <= hi[2]; ++z)
+                for (int64_t y = lo[1]; y <= hi[1]; ++y)
+                    for (int64_t x = lo[0]; x <= hi[0]; ++x) jarr[cell(&g, x, y, z) + g.M * nw] += v;
+        }
+    }
+    g_free(&g);
+    return 0;
+}
+
 int og_field_arr2vec(const og_desc *d, const oc64 *jarr, oc64 *vec) {
     G g; int rc = g_build(d, &g); if (rc) return rc;
     for (int c = 0; c < 3; ++c)

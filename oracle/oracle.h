@@ -64,6 +64,8 @@ int og_distweights(double c, int gt, const double bounds[2], const double *l,
                    const double *dl, int64_t N, int isbloch, int64_t ind[2], double wt[2]);
 int og_add_pointsrc(const og_desc *d, int ft, const double c[3], const double p[3],
                     double Idr_re, double Idr_im, oc64 *jarr /* (Nx,Ny,Nz,3) */);
+int og_add_planesrc(const og_desc *d, int ft, int nn /* normal axis 0..2 */, double c,
+                    const double p[3], double Jdn_re, double Jdn_im, oc64 *jarr /* (Nx,Ny,Nz,3) */);
 int og_field_arr2vec(const og_desc *d, const oc64 *jarr, oc64 *vec);
 
 /* sparse assembly; CSC arrays are malloc'ed by the oracle and freed by og_csc_free */

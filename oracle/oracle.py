@@ -137,6 +137,22 @@ def add_pointsrc(spec, ft, c, p, Idr, jarr):
         raise ValueError(f"point source rejected (code {rc})")
 
 
+def add_planesrc(spec, ft, n, c, p, Jdn, jarr):
+    """planesrc.jl:41-79 into jarr (Nx,Ny,Nz,3); n is a Cartesian unit normal."""
+    assert jarr.flags.f_contiguous and jarr.dtype == np.complex128
+    n = np.asarray(n, dtype=np.float64)
+    n = n / np.sqrt(np.sum(n * n))
+    nn = int(np.flatnonzero(n == 1)[0])
+    p = np.asarray(p, dtype=np.float64)
+    p = p / np.sqrt(np.sum(p * p))
+    pp = (C.c_double * 3)(*[float(v) for v in p])
+    Jdn = complex(Jdn)
+    rc = lib().og_add_planesrc(C.byref(_desc(spec)), C.c_int(int(ft)), C.c_int(nn), C.c_double(float(c)), pp,
+                               C.c_double(Jdn.real), C.c_double(Jdn.imag), _ptr(jarr))
+    if rc != 0:
+        raise ValueError(f"plane source rejected (code {rc})")
+
+
 def field_arr2vec(spec, jarr):
     d = _desc(spec)
     v = np.zeros(jarr.size, dtype=np.complex128)

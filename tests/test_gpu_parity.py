@@ -231,6 +231,46 @@ def test_apply_matches_csc(case, full, mu, pkg, O):
     op.close()
 
 
+@pytest.mark.parametrize("case", ["uniform_pml", "nonuniform_bloch", "mixed_pml_bloch", "cmplast", "wide"])
+@pytest.mark.parametrize("mu", [False, True])
+def test_h_from_e_e_from_h(case, mu, pkg, O):
+    """h_from_e / e_from_h (model.jl:276-284) against the oracle's Cₑ, Cₘ, Pε, Pμ
+    CSC products; and the identity e_from_h(h_from_e(e)) = e + ω⁻² Pε⁻¹ A e."""
+    import scipy.sparse.linalg as spla
+    m = make(pkg, case)
+    s = spec_of(m)
+    eps, muu = rand_params(m, 9, mu=mu)
+    om = 0.55 - 0.02j
+    say("field convert", case, mu)
+    Ce, Cm = (O.to_scipy(O.curl(s, w)).tocsc() for w in (0, 1))
+    Pe, Pm = (O.to_scipy(O.paramop(s, w, a)).tocsc() for w, a in ((0, eps), (1, muu)))
+    A = O.to_scipy(O.create_A(s, om, eps, muu))
+    op = pkg.MaxwellOperator(m, om, εarr=eps, μarr=muu)
+    rng = np.random.default_rng(4)
+    n = 3 * m.grid.M
+    cr = lambda: rng.normal(size=n) + 1j * rng.normal(size=n)
+    e, jm, je = cr(), cr(), cr()
+    href = (1j / om) * spla.spsolve(Pm, Ce @ e + jm)
+    h = op.h_from_e(e, jm)
+    assert np.linalg.norm(h - href) <= 1e-12 * np.linalg.norm(href)
+    eref = (-1j / om) * spla.spsolve(Pe, Cm @ h - je)
+    e2 = op.e_from_h(h, je)
+    assert np.linalg.norm(e2 - eref) <= 1e-12 * np.linalg.norm(eref)
+    rt = op.e_from_h(op.h_from_e(e))
+    ident = e + spla.spsolve(Pe, A @ e) / om**2
+    assert np.linalg.norm(rt - ident) <= 1e-11 * np.linalg.norm(ident)
+    op.close()
+
+
+def test_e_from_h_rejects_full_eps(pkg):
+    m = make(pkg, "uniform_pml")
+    eps, muu = rand_params(m, 9, full=True)
+    op = pkg.MaxwellOperator(m, 0.5, εarr=eps, μarr=muu)
+    with pytest.raises(pkg.MfdUnsupportedError):
+        op.e_from_h(np.zeros(3 * m.grid.M, dtype=complex))
+    op.close()
+
+
 def c1_like(pkg, N=12, npml=3, r=2.5, lam=8.0):
     from maxwellfdfd import workloads as W
     m, om = W.sphere_dipole(N=N, npml=npml, r=r, lam=lam)

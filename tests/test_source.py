@@ -134,3 +134,74 @@ def test_c1_dipole_closed_form(O):
     nz = np.argwhere(j != 0)
     assert sorted(map(tuple, nz)) == [(16, 16, 15, 2), (16, 16, 16, 2)]
     assert j[16, 16, 15, 2] == 0.5 and j[16, 16, 16, 2] == 0.5
+
+
+def _planesrc_oracle(O, lprim, n, c, p, Jdn=1.0, bloch=(True, True, True)):
+    N = tuple(len(a) - 1 for a in lprim)
+    j = np.zeros(N + (3,), dtype=np.complex128, order="F")
+    O.add_planesrc(_spec(lprim, bloch), 0, n, c, p, Jdn, j)
+    return j
+
+
+def _planesrc_capi(pkg, lprim, n, c, p, Jdn=1.0, bloch=(True, True, True)):
+    m = pkg.ModelFull(pkg.Grid(lprim, bloch))
+    pkg.add_srce(m, pkg.PlaneSrc(n, c, p, Jdn))
+    return m.jₑarr
+
+
+@pytest.mark.parametrize("impl", ["oracle", "capi"])
+def test_planesrc_reference_testset(impl, O, pkg):
+    """test/source.jl:123-170: x-polarised source in the z = 0 plane, Bloch grid."""
+    make = (lambda lp: _planesrc_oracle(O, lp, [0, 0, 1], 0.0, [1, 0, 0])) if impl == "oracle" else \
+        (lambda lp: _planesrc_capi(pkg, lp, [0, 0, 1], 0.0, [1, 0, 0]))
+    lp = [np.arange(-10, 11.0)] * 3
+    j = make(lp)
+    assert np.max(np.abs(j)) == 1.0           # J = K / ∆z (:139)
+    assert np.all(j[..., 1] == 0) and np.all(j[..., 2] == 0)   # (:140-141)
+    lpf = [np.arange(-10, 10.25, 0.5)] * 3
+    jf = make(lpf)
+    # total current through an x-normal cross section is resolution independent (:151)
+    assert np.sum(j[0, :, :, 0]) * 1.0 == pytest.approx(np.sum(jf[0, :, :, 0]) * 0.25, rel=1e-14)
+    assert np.max(np.abs(jf)) == 2.0          # (:152)
+    # non-uniform z (:155-168), seeded
+    rng = np.random.default_rng(11)
+    z = np.sort(rng.random(21)) * 20
+    z -= z.mean()
+    lpn = [np.arange(-10, 11.0), np.arange(-10, 11.0), z]
+    jn = make(lpn)
+    # ∆l at primal points = diff of the dual vertices including the Bloch ghost
+    def dprim(a):
+        ld = 0.5 * (a[:-1] + a[1:])
+        return np.diff(np.concatenate(([ld[-1] - (a[-1] - a[0])], ld)))
+    area = dprim(lpn[1])[:, None] * dprim(lpn[2])[None, :]
+    assert np.sum(j[0, :, :, 0]) == pytest.approx(np.sum(jn[0, :, :, 0] * area), rel=1e-12)
+
+
+def test_planesrc_capi_matches_oracle(O, pkg):
+    rng = np.random.default_rng(5)
+    lp = [np.cumsum(np.concatenate(([0.0], rng.uniform(0.5, 1.5, n)))) for n in (9, 7, 8)]
+    N = tuple(len(a) - 1 for a in lp)
+    for bloch in ((False, False, False), (True, False, True)):
+        for nn in range(3):
+            n = [0, 0, 0]
+            n[nn] = 2.5
+            for _ in range(3):
+                c = rng.uniform(lp[nn][0], lp[nn][-1])
+                p = rng.normal(size=3)
+                jo = _planesrc_oracle(O, lp, n, c, p, 0.5 + 2j, bloch)
+                jc = _planesrc_capi(pkg, lp, n, c, p, 0.5 + 2j, bloch)
+                np.testing.assert_allclose(jc, jo, rtol=1e-14, atol=1e-15)
+                # each component lands on at most two planes normal to nn
+                for w in range(3):
+                    planes = np.unique(np.argwhere(jo[..., w] != 0)[:, nn])
+                    assert len(planes) <= 2
+
+
+def test_planesrc_rejects(pkg):
+    with pytest.raises(pkg.MfdArgumentError):
+        pkg.PlaneSrc([1, 1, 0], 0.0, [0, 0, 1])
+    m = pkg.ModelFull(pkg.Grid([np.arange(5.0)] * 3, (False, False, False)))
+    with pytest.raises(pkg.MfdArgumentError):
+        pkg.add_srce(m, pkg.PlaneSrc([0, 0, -1], 1.0, [1, 0, 0]))   # no +1 entry (planesrc.jl:42)
+    with pytest.raises(pkg.MfdArgumentError):
+        pkg.add_srce(m, pkg.PlaneSrc([0, 0, 1], 9.0, [1, 0, 0]))    # outside the domain (source.jl:214)
