@@ -25,9 +25,9 @@ HBM_PEAK_GBPS = 8000.0
 METRIC = "solve time to rel. residual 1e-8 + curl-curl apply GB/s; mixed vs FP64 on GPU"
 KERNELS = ("stencil", "upd_r", "upd_dzq")
 KERNEL_NAMES = {  # device symbols as rocprofv3 lists them (prefix match)
-    "stencil": "k_apply_tiled<{R}, 1, 1",
-    "upd_r": "k_upd_r<{R}>",
-    "upd_dzq": "k_upd_dzq<{R}>",
+    "stencil": "k_apply_tiled<{R}, 1, 2",
+    "upd_r": "k_upd_r<{R},",
+    "upd_dzq": "k_upd_dzq<{R},",
 }
 
 
@@ -53,6 +53,13 @@ def parse():
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_c2.json"))
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the per-kernel timing launches and a 256 MiB device copy (for hardware-counter runs)")
+    ap.add_argument("--workload", default="random-boxes",
+                    choices=["random-boxes", "batched-rhs", "uneven-batch", "adjoint"],
+                    help="random-boxes: the metric line; the others time the batched-RHS, uneven-problem "
+                         "and adjoint configurations")
+    ap.add_argument("--nrhs", type=int, default=4, help="batched-rhs: right-hand sides")
+    ap.add_argument("--count", type=int, default=32, help="uneven-batch: problems")
+    ap.add_argument("--voxels", type=int, default=4, help="adjoint: finite-difference voxels")
     return ap.parse_args()
 
 
@@ -108,12 +115,144 @@ def kernel_table(op, cells, iters=50):
     return out
 
 
+def run_batched_rhs(a):
+    """N³ random boxes (the 40-box generator) with nrhs seeded dipoles solved in
+    one batched FP64 launch, against one single-RHS solve of the first dipole."""
+    import numpy as np
+    import torch
+    import maxwellfdfd as M
+    from maxwellfdfd import workloads as W
+    N = a.N
+    m, om = W.random_boxes(N=N, npml=10, nbox=40, lam=24.0, seed=1, ndip=0)
+    M.calc_matparams(m)
+    B = W.dipole_rhs(m, om, W.random_dipoles(N, 10, a.nrhs, 2024))
+    op = M.MaxwellOperator(m, om)
+    op.autotune(a.nrhs, 5)
+    dev = torch.device("cuda:0")
+    b_d = torch.from_numpy(B).to(dev)
+    e_d = torch.zeros_like(b_d)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = op.solve_dev(b_d.data_ptr(), e_d.data_ptr(), a.nrhs, tol=1e-8)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    e1 = torch.zeros_like(b_d[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st1 = op.solve_dev(b_d[0].data_ptr(), e1.data_ptr(), 1, tol=1e-8)
+    torch.cuda.synchronize()
+    t_one = time.perf_counter() - t0
+    err = float(torch.linalg.vector_norm(e_d[0] - e1) / torch.linalg.vector_norm(e1))
+    op.close()
+    return {"workload": f"batched-rhs {N}^3 random boxes, {a.nrhs} dipoles, FP64 SQMR to 1e-8",
+            "batched_s": round(t_batch, 3), "iterations_max": int(st.iterations), "relres_max": st.relres,
+            "single_rhs_s": round(t_one, 3), "single_rhs_iterations": int(st1.iterations),
+            "s_per_rhs_batched": round(t_batch / a.nrhs, 3), "rel_diff_rhs0_vs_single": err}
+
+
+def run_uneven_batch(a):
+    """`count` independent problems (N ~ U{48..128}, own boxes, own ω) in one
+    lockstep graph-launched batched FP64 solve, against the same problems solved
+    one after another."""
+    import numpy as np
+    import torch
+    import maxwellfdfd as M
+    from maxwellfdfd import workloads as W
+    probs = W.uneven_problems(count=a.count, nmin=48, nmax=128, npml=10, nbox=40, seed=7)
+    dev = torch.device("cuda:0")
+    ops, bds, eds, cells = [], [], [], 0
+    for m, om in probs:
+        M.calc_matparams(m)
+        je, jm = M.create_srcs(m)
+        b = M.create_b(M.EE, om, m, je, jm)
+        op = M.MaxwellOperator(m, om)
+        op.autotune(1, 5)
+        ops.append(op)
+        bds.append(torch.from_numpy(b).to(dev))
+        eds.append(torch.zeros_like(bds[-1]))
+        cells += m.grid.M
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = ops[0].solve_batch_dev(ops[1:], [x.data_ptr() for x in bds], [x.data_ptr() for x in eds], tol=1e-8)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    its = []
+    for op, bd, ed in zip(ops, bds, eds):
+        ed.zero_()
+        its.append(int(op.solve_dev(bd.data_ptr(), ed.data_ptr(), 1, tol=1e-8).iterations))
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    for op in ops:
+        op.close()
+    return {"workload": f"uneven-batch: {a.count} problems, N ~ U{{48..128}}, omega ~ U[2pi/30, 2pi/16], "
+                        f"FP64 SQMR to 1e-8",
+            "sizes": [int(m.grid.N[0]) for m, _ in probs], "cells_total": int(cells),
+            "batched_s": round(t_batch, 3), "sequential_s": round(t_seq, 3),
+            "speedup_batched_vs_sequential": round(t_seq / t_batch, 3),
+            "iterations": [int(s.iterations) for s in st], "converged": all(bool(s.converged) for s in st),
+            "iterations_sequential": its}
+
+
+def run_adjoint(a):
+    """128×128×64 adjoint case: forward solve, adjoint solve and ∂f/∂ε, with
+    central finite differences (h = 1e-4, warm-started solves) on seeded design voxels."""
+    import numpy as np
+    import maxwellfdfd as M
+    from maxwellfdfd import workloads as W
+    m, om, b, mon, sl = W.adjoint_design()
+    op = M.MaxwellOperator(m, om)
+    op.autotune(1, 5)
+    t0 = time.perf_counter()
+    e, pullback = op.solve_with_pullback(b, tol=1e-11)
+    t_fwd = time.perf_counter() - t0
+    g = np.zeros_like(e)
+    g[mon] = np.conj(e[mon])
+    t0 = time.perf_counter()
+    grad = pullback(g)
+    t_adj = time.perf_counter() - t0
+    op.close()
+    N = m.grid.N
+    rng = np.random.default_rng(1)
+    h = 1e-4
+    errs = []
+    for _ in range(a.voxels):
+        c = int(rng.integers(3))
+        i, j, k = (int(rng.integers(s.start, s.stop)) for s in sl)
+        fs = []
+        for sgn in (1, -1):
+            ea = m.εarr.copy()
+            ea[i, j, k, c, c] += sgn * h
+            o2 = M.MaxwellOperator(m, om, εarr=ea)
+            x, _ = o2.solve(b, tol=1e-13, x0=e)
+            o2.close()
+            fs.append(np.sum(np.abs(x[mon]) ** 2))
+        fd = (fs[0] - fs[1]) / (2 * h)
+        dof = c + 3 * (i + N[0] * (j + N[1] * k))
+        errs.append(abs(grad[dof] - fd) / max(abs(fd), 1e-300))
+    return {"workload": "adjoint: 128x128x64, 10-cell PML, 64x64x16 design region eps = 1 + 11 sigmoid(G*xi), "
+                        "z-dipole, |E_z|^2 over a 4x4x1 monitor",
+            "forward_s": round(t_fwd, 3), "adjoint_and_grad_s": round(t_adj, 3),
+            "fd_voxels": a.voxels, "fd_rel_err_max": max(errs), "fd_rel_err": errs}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); this build has no multi-GPU path.",
               file=sys.stderr)
         sys.exit(2)
+    if a.workload != "random-boxes":
+        if a.workload == "batched-rhs":
+            if a.N == 128:
+                a.N = 160
+            out = run_batched_rhs(a)
+        elif a.workload == "uneven-batch":
+            out = run_uneven_batch(a)
+        else:
+            out = run_adjoint(a)
+        print(json.dumps(out))
+        return
     import numpy as np
     import torch
     import maxwellfdfd as M

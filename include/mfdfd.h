@@ -37,7 +37,9 @@ enum {
 enum { MFD_EE = 0, MFD_HH = 1 };      /* FieldType */
 enum { MFD_PRIM = 0, MFD_DUAL = 1 };  /* GridType */
 enum { MFD_FP64 = 0, MFD_MIXED = 1 }; /* solve precision */
-enum { MFD_SQMR = 0, MFD_COCG = 1 };  /* Krylov method (MFD_COCG: with reliable updates) */
+/* Krylov method: SQMR and COCG (with reliable updates) need the complex-symmetric
+ * operator of a grid without complex Bloch phases; BiCGSTAB serves the rest. */
+enum { MFD_SQMR = 0, MFD_COCG = 1, MFD_BICGSTAB = 2 };
 
 typedef struct { double re, im; } mfd_c64;
 
@@ -72,7 +74,7 @@ typedef struct {
     double tol;               /* relative residual ||b-Ae||/||b|| in FP64 (1e-8) */
     int64_t maxit;            /* total Krylov iterations allowed per RHS */
     int32_t precision;        /* MFD_FP64 or MFD_MIXED (FP32 inner, FP64 refinement) */
-    int32_t method;           /* MFD_SQMR, MFD_COCG, or -1 (SQMR) */
+    int32_t method;           /* MFD_SQMR, MFD_COCG, MFD_BICGSTAB, or -1 (SQMR; BiCGSTAB with a complex Bloch phase) */
     double inner_tol;         /* mixed: tightest inner reduction per FP32 cycle (1e-5) */
     int32_t check_every;      /* iterations between host convergence checks (32) */
     int32_t use_graph;        /* capture a chunk of iterations in a hipGraph */
@@ -129,15 +131,18 @@ int mfd_paramop_nnz(mfd_ctx *ctx, const mfd_grid *g, int which, const mfd_c64 *p
 int mfd_paramop_csc(mfd_ctx *ctx, const mfd_grid *g, int which, const mfd_c64 *parr,
                     int64_t *colptr, int64_t *rowval, mfd_c64 *nzval);
 
-/* create_A (model.jl:225-246), A = Cₘ (Pμ \ Cₑ) − ω² Pε for ft = EE. */
+/* create_A (model.jl:225-246): A = Cₘ (Pμ \ Cₑ) − ω² Pε for ft = EE,
+ * A = Cₑ (Pε \ Cₘ) − ω² Pμ for ft = HH (the divided operator must be diagonal). */
 int mfd_A_nnz(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr,
               const mfd_c64 *mu_arr, int64_t *nnz);
 int mfd_A_csc(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr,
               const mfd_c64 *mu_arr, int64_t *colptr, int64_t *rowval, mfd_c64 *nzval);
 
-/* create_b (model.jl:251-274) for ft = EE: b = −Cₘ (Pμ \ jₘ) − iω jₑ. */
-int mfd_create_b(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *mu_arr,
-                 const mfd_c64 *je, const mfd_c64 *jm, mfd_c64 *b);
+/* create_b (model.jl:251-274): b = −Cₘ (Pμ \ jₘ) − iω jₑ for ft = EE (uses
+ * mu_arr), b = Cₑ (Pε \ jₑ) − iω jₘ for ft = HH (uses eps_arr); boundft = EE
+ * on every axis. */
+int mfd_create_b(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr,
+                 const mfd_c64 *mu_arr, const mfd_c64 *je, const mfd_c64 *jm, mfd_c64 *b);
 
 /* Matrix-free operator for the A of create_A (model.jl:235-237), device
  * resident.  eps_arr/mu_arr as produced by create_paramops (host). */
@@ -157,11 +162,13 @@ int mfd_op_apply_dev(mfd_op *op, const void *x, void *y, int64_t nrhs, int prec)
  * counterpart: the reference has no solver (model.jl stops at create_A). */
 int mfd_op_set_graddiv(mfd_op *op, int enable);
 /* Tile-shape autotuner for the stencil (rows per block TY, z-planes per block
- * KZ): times each candidate for nrhs right-hand sides and keeps the fastest
- * per precision.  Outputs may be null.  No reference counterpart. */
-int mfd_op_autotune(mfd_op *op, int64_t nrhs, int iters, int tile64[2], int tile32[2], double ms[2]);
-/* Fix the tile shape by hand: TY in {1,2,4} (0 = cache-reliant direct kernel), KZ >= 1. */
-int mfd_op_set_tile(mfd_op *op, int prec, int ty, int kz);
+ * KZ, tile order: 1 = each XCD gets a contiguous run of tiles, 0 = dispatch
+ * order): times each candidate for nrhs right-hand sides and keeps the fastest
+ * per precision.  Outputs {TY, KZ, order} may be null.  No reference counterpart. */
+int mfd_op_autotune(mfd_op *op, int64_t nrhs, int iters, int tile64[3], int tile32[3], double ms[2]);
+/* Fix the tile shape by hand: TY in {1,2,4} (0 = cache-reliant direct kernel),
+ * KZ >= 1, xcd_order 0 or 1. */
+int mfd_op_set_tile(mfd_op *op, int prec, int ty, int kz, int xcd_order);
 /* Average device time (ms) of one apply of the solver's stencil kernel
  * (MFD_FP64: A; MFD_MIXED: the complex-FP32 inner operator) */
 int mfd_op_bench_apply(mfd_op *op, int64_t nrhs, int prec, int iters, double *ms_per_apply);

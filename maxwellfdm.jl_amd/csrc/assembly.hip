@@ -193,28 +193,32 @@ __global__ void k_param(DGrid<double> g, int which, const c64 *parr, const int *
     for (int x = 0; x < L.n; ++x) { rowval[o + x] = L.e[x].row + 1; nzval[o + x] = L.e[x].v; }
 }
 
-// Column of A = Cₘ (Pμ \ Cₑ) − ω² Pε.  The product is accumulated in the
-// order of Gustavson's algorithm (Cₑ column entries by row, then Cₘ column
-// entries by row), then rows are sorted; the ω² term is subtracted with
-// zero results dropped (skipped when ω = 0, as model.jl:237 does).
+// Column of A = Cₘ (Pμ \ Cₑ) − ω² Pε (ft = EE, model.jl:235-237) or
+// A = Cₑ (Pε \ Cₘ) − ω² Pμ (ft = HH, model.jl:238-240).  The product is
+// accumulated in the order of Gustavson's algorithm (inner-factor column
+// entries by row, then outer-factor column entries by row), then rows are
+// sorted; the ω² term is subtracted with zero results dropped (skipped when
+// ω = 0, as the reference does).  `dv` is the diagonal divisor array (μ for
+// EE, ε for HH) and `pw` the parameter array of the ω² term.
 template <bool FILL>
-__global__ void k_A(DGrid<double> g, const c64 *eps, const c64 *mu, const int *full, c64 om2, int omz, int64_t n,
-                    int64_t *cnt, const int64_t *colptr, int64_t *rowval, c64 *nzval) {
+__global__ void k_A(DGrid<double> g, int ft, const c64 *pw, const c64 *dv, const int *full, c64 om2, int omz,
+                    int64_t n, int64_t *cnt, const int64_t *colptr, int64_t *rowval, c64 *nzval) {
     int64_t col = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (col >= n) return;
+    const int inner = ft == EE ? 0 : 1;   // Cₑ first for EE, Cₘ first for HH
     int w;
     int64_t p;
     col2cp(g, col, w, p);
     EList<4> ce;
-    curl_col(g, 0, w, p, ce);
+    curl_col(g, inner, w, p, ce);
     EList<16> acc;
     for (int k = 0; k < ce.n; ++k) {
         int u;
         int64_t r;
         col2cp(g, ce.e[k].row, u, r);
-        c64 b = cdiv(ce.e[k].v, mu[r + g.M * (u + 3 * u)]);
+        c64 b = cdiv(ce.e[k].v, dv[r + g.M * (u + 3 * u)]);
         EList<4> cm;
-        curl_col(g, 1, u, r, cm);
+        curl_col(g, 1 - inner, u, r, cm);
         for (int x = 0; x < cm.n; ++x) {
             c64 prod = cm.e[x].v * b;
             int f = -1;
@@ -234,7 +238,7 @@ __global__ void k_A(DGrid<double> g, const c64 *eps, const c64 *mu, const int *f
         }
     } else {
         EList<9> pe;
-        param_col(g, 0, eps, *full, w, p, pe);
+        param_col(g, inner, pw, *full, w, p, pe);
         int a = 0, b = 0;
         while (a < acc.n || b < pe.n) {
             int64_t ra = a < acc.n ? acc.e[a].row : INT64_MAX, rb = b < pe.n ? pe.e[b].row : INT64_MAX;
@@ -360,27 +364,28 @@ int param_csc(mfd_ctx *ctx, const DevGrid &G, int which, const mfd_c64 *parr, in
                      [&](int64_t *cp, int64_t *rv, c64 *nz) { k_param<true><<<nblk(n, 256), 256, 0, s>>>(g, which, pa, full, n, nullptr, cp, rv, nz); });
 }
 
-int A_csc(mfd_ctx *ctx, const DevGrid &G, mfd_c64 omega, const mfd_c64 *eps, const mfd_c64 *mu, int64_t *nnz,
-          int64_t *colptr, int64_t *rowval, mfd_c64 *nzval) {
+int A_csc(mfd_ctx *ctx, const DevGrid &G, int ft, mfd_c64 omega, const mfd_c64 *eps, const mfd_c64 *mu,
+          int64_t *nnz, int64_t *colptr, int64_t *rowval, mfd_c64 *nzval) {
     const DGrid<double> &g = G.d64;
     int64_t n = 3 * g.M;
     hipStream_t s = ctx->stream;
-    DevArray E, U;
-    int mufull = 0;
-    MFD_TRY(upload_param(ctx, G, mu, U, &mufull));
-    if (mufull) {
-        set_error("create_A: Pμ must be diagonal (Pμ \\ Cₑ), model.jl:236");
+    DevArray Dv, Pw;   // divisor (diagonal) and ω²-term parameter arrays
+    int dfull = 0;
+    MFD_TRY(upload_param(ctx, G, ft == EE ? mu : eps, Dv, &dfull));
+    if (dfull) {
+        set_error(ft == EE ? "create_A: Pμ must be diagonal (Pμ \\ Cₑ), model.jl:236"
+                           : "create_A: Pε must be diagonal (Pε \\ Cₘ), model.jl:239");
         return MFD_ERR_UNSUPPORTED;
     }
-    MFD_TRY(upload_param(ctx, G, eps, E, nullptr));
-    const c64 *ea = (const c64 *)E.p, *ua = (const c64 *)U.p;
-    const int *full = (const int *)((const char *)E.p + sizeof(c64) * 9 * g.M);
+    MFD_TRY(upload_param(ctx, G, ft == EE ? eps : mu, Pw, nullptr));
+    const c64 *pa = (const c64 *)Pw.p, *da = (const c64 *)Dv.p;
+    const int *full = (const int *)((const char *)Pw.p + sizeof(c64) * 9 * g.M);
     c64 om = mk<double>(omega.re, omega.im);
     c64 om2 = om * om;
     int omz = (omega.re == 0.0 && omega.im == 0.0);
     return csc_drive(s, n, nnz, colptr, rowval, nzval,
-                     [&](int64_t *cnt) { k_A<false><<<nblk(n, 128), 128, 0, s>>>(g, ea, ua, full, om2, omz, n, cnt, nullptr, nullptr, nullptr); },
-                     [&](int64_t *cp, int64_t *rv, c64 *nz) { k_A<true><<<nblk(n, 128), 128, 0, s>>>(g, ea, ua, full, om2, omz, n, nullptr, cp, rv, nz); });
+                     [&](int64_t *cnt) { k_A<false><<<nblk(n, 128), 128, 0, s>>>(g, ft, pa, da, full, om2, omz, n, cnt, nullptr, nullptr, nullptr); },
+                     [&](int64_t *cp, int64_t *rv, c64 *nz) { k_A<true><<<nblk(n, 128), 128, 0, s>>>(g, ft, pa, da, full, om2, omz, n, nullptr, cp, rv, nz); });
 }
 
 }  // namespace mfd

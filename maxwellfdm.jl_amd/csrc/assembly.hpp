@@ -18,7 +18,8 @@ int curl_csc(mfd_ctx *ctx, const DevGrid &G, int which, int64_t *nnz, int64_t *c
              mfd_c64 *nzval);
 int param_csc(mfd_ctx *ctx, const DevGrid &G, int which, const mfd_c64 *parr, int64_t *nnz, int64_t *colptr,
               int64_t *rowval, mfd_c64 *nzval);
-int A_csc(mfd_ctx *ctx, const DevGrid &G, mfd_c64 omega, const mfd_c64 *eps, const mfd_c64 *mu, int64_t *nnz,
-          int64_t *colptr, int64_t *rowval, mfd_c64 *nzval);
+// ft = EE: A = Cₘ (Pμ \ Cₑ) − ω² Pε; ft = HH: A = Cₑ (Pε \ Cₘ) − ω² Pμ
+int A_csc(mfd_ctx *ctx, const DevGrid &G, int ft, mfd_c64 omega, const mfd_c64 *eps, const mfd_c64 *mu,
+          int64_t *nnz, int64_t *colptr, int64_t *rowval, mfd_c64 *nzval);
 
 }  // namespace mfd

@@ -11,7 +11,7 @@ int op_destroy_impl(mfd_op *op);
 int op_apply_impl(mfd_op *op, const mfd_c64 *x, mfd_c64 *y, int64_t nrhs);
 int op_apply_dev_impl(mfd_op *op, const void *x, void *y, int64_t nrhs, int prec);
 int op_set_graddiv_impl(mfd_op *op, int enable);
-int op_set_tile_impl(mfd_op *op, int prec, int ty, int kz);
+int op_set_tile_impl(mfd_op *op, int prec, int ty, int kz, int xcd);
 int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best32, double *ms_out);
 int op_bench_apply_impl(mfd_op *op, int64_t nrhs, int prec, int iters, double *ms);
 int op_bench_iteration_impl(mfd_op *op, int64_t nrhs, int prec, int iters, double *ms);
@@ -25,7 +25,7 @@ int solve_batch_dev_impl(int count, mfd_op *const *ops, const void *const *b, vo
 int field_convert_impl(mfd_op *op, int which, const mfd_c64 *in, const mfd_c64 *src, mfd_c64 *out);
 int solve_adjoint_impl(mfd_op *op, const mfd_c64 *g, mfd_c64 *lam, const mfd_solve_opts *o, mfd_solve_stats *st);
 int adjoint_grad_impl(mfd_op *op, const mfd_c64 *e, const mfd_c64 *lam, double *grad);
-int create_b_impl(mfd_ctx *ctx, const mfd::DevGrid &G, mfd_c64 omega, const mfd_c64 *mu_arr, const mfd_c64 *je,
+int create_b_impl(mfd_ctx *ctx, const mfd::DevGrid &G, int ft, mfd_c64 omega, const mfd_c64 *par, const mfd_c64 *je,
                   const mfd_c64 *jm, mfd_c64 *b);
 
 using namespace mfd;
@@ -228,8 +228,8 @@ int mfd_paramop_csc(mfd_ctx *ctx, const mfd_grid *g, int which, const mfd_c64 *p
 }
 
 static int ft_check(int ft) {
-    if (ft == MFD_EE) return MFD_OK;
-    set_error(ft == MFD_HH ? "ft = HH is not implemented (E-field formulation only)" : "ft is unsupported");
+    if (ft == MFD_EE || ft == MFD_HH) return MFD_OK;
+    set_error("ft is unsupported");   // model.jl:241-242, 269-270
     return MFD_ERR_UNSUPPORTED;
 }
 
@@ -239,7 +239,7 @@ int mfd_A_nnz(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_
     MFD_TRY(ft_check(ft));
     DevGrid G;
     MFD_TRY(G.create(g, ctx->stream));
-    return A_csc(ctx, G, omega, eps_arr, mu_arr, nnz, nullptr, nullptr, nullptr);
+    return A_csc(ctx, G, ft, omega, eps_arr, mu_arr, nnz, nullptr, nullptr, nullptr);
 }
 
 int mfd_A_csc(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr, const mfd_c64 *mu_arr,
@@ -249,16 +249,18 @@ int mfd_A_csc(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_
     DevGrid G;
     MFD_TRY(G.create(g, ctx->stream));
     int64_t nnz;
-    return A_csc(ctx, G, omega, eps_arr, mu_arr, &nnz, colptr, rowval, nzval);
+    return A_csc(ctx, G, ft, omega, eps_arr, mu_arr, &nnz, colptr, rowval, nzval);
 }
 
-int mfd_create_b(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *mu_arr, const mfd_c64 *je,
-                 const mfd_c64 *jm, mfd_c64 *b) {
+int mfd_create_b(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr,
+                 const mfd_c64 *mu_arr, const mfd_c64 *je, const mfd_c64 *jm, mfd_c64 *b) {
     CHECK_CTX(ctx);
     MFD_TRY(ft_check(ft));
+    for (int a = 0; a < 3; ++a)
+        if (g->boundft[a] != MFD_EE) { set_error("create_b: boundft must be EE on every axis"); return MFD_ERR_UNSUPPORTED; }
     DevGrid G;
     MFD_TRY(G.create(g, ctx->stream));
-    return create_b_impl(ctx, G, omega, mu_arr, je, jm, b);
+    return create_b_impl(ctx, G, ft, omega, ft == MFD_EE ? mu_arr : eps_arr, je, jm, b);
 }
 
 int mfd_op_create(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr,
@@ -283,11 +285,11 @@ int mfd_op_set_graddiv(mfd_op *op, int enable) {
     CHECK_CTX(op->ctx);
     return op_set_graddiv_impl(op, enable);
 }
-int mfd_op_set_tile(mfd_op *op, int prec, int ty, int kz) {
+int mfd_op_set_tile(mfd_op *op, int prec, int ty, int kz, int xcd_order) {
     if (!op) { set_error("null operator"); return MFD_ERR_ARG; }
-    return op_set_tile_impl(op, prec, ty, kz);
+    return op_set_tile_impl(op, prec, ty, kz, xcd_order);
 }
-int mfd_op_autotune(mfd_op *op, int64_t nrhs, int iters, int tile64[2], int tile32[2], double ms[2]) {
+int mfd_op_autotune(mfd_op *op, int64_t nrhs, int iters, int tile64[3], int tile32[3], double ms[2]) {
     if (!op || nrhs < 1 || iters < 1) { set_error("bad argument"); return MFD_ERR_ARG; }
     CHECK_CTX(op->ctx);
     return op_autotune_impl(op, nrhs, iters, tile64, tile32, ms);

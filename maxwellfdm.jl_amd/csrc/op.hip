@@ -229,10 +229,18 @@ __global__ __launch_bounds__(BS) void k_fin_dot(const double *part, int nb, Scal
     if (S.conv) { S.done = 1; return; }
     S.sig_re = tot[0];
     S.sig_im = tot[1];
-    if (tot[0] == 0.0 && tot[1] == 0.0) { S.breakdown = 1; S.conv = 1; S.alpha_re = S.alpha_im = 0; return; }
+    if (tot[0] == 0.0 && tot[1] == 0.0) {   // breakdown: no step (the stencil already applied the pending update)
+        S.breakdown = 1;
+        S.conv = 1;
+        S.alpha_re = S.alpha_im = 0;
+        S.apend_re = S.apend_im = 0;
+        return;
+    }
     c64 a = cdiv(mk<double>(S.rho_re, S.rho_im), mk<double>(tot[0], tot[1]));
     S.alpha_re = a.re;
     S.alpha_im = a.im;
+    S.apend_re = a.re;
+    S.apend_im = a.im;
 }
 
 // after the stencil with EPI_RESID: ||b − A x||² per (unmasked) RHS
@@ -762,27 +770,31 @@ static void launch_finish(hipStream_t s, int nb, int64_t nrhs, double *part, Sca
 // launch one operator apply
 template <typename R, int MODE, int EPI>
 static void launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int64_t nrhs, double *part,
-                         Scal *scal, double *rn, const int *mask = nullptr) {
+                         Scal *scal, double *rn, const int *mask = nullptr, hipEvent_t after_main = nullptr) {
     const DGrid<R> &g = op->G.view<R>();
     const OpData<R> &od = opd<R>(op);
     int64_t vs = 3 * g.M;
     hipStream_t s = op->stream;
     int ty = std::is_same<R, double>::value ? op->tile64[0] : op->tile32[0];
     int kz = std::is_same<R, double>::value ? op->tile64[1] : op->tile32[1];
+    int xr = std::is_same<R, double>::value ? op->tile64[2] : op->tile32[2];
+    if (MODE == MODE_P && op->gd && ty == 8) ty = 4;   // the grad-div tile keeps two more LDS planes
     if (!op->full && !op->has_mu && ty > 0) {
         int nx = (g.N[0] + 63) / 64, ny = (g.N[1] + ty - 1) / ty;
         int nch = (g.N[2] + kz - 1) / kz;
         while ((int64_t)nx * ny * nch > NPART) { kz *= 2; nch = (g.N[2] + kz - 1) / kz; }
         dim3 grid(nx, ny, (unsigned)(nch * nrhs));
         if (MODE == MODE_P && op->gd) {
-            if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, true><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask);
-            else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, true><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask);
-            else k_apply_tiled<R, MODE, EPI, 1, true><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask);
+            if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, true><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, true><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else k_apply_tiled<R, MODE, EPI, 1, true><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
         } else {
-            if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, false><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask);
-            else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, false><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask);
-            else k_apply_tiled<R, MODE, EPI, 1, false><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask);
+            if (ty == 8) k_apply_tiled<R, MODE, EPI, 8, false><<<grid, 512, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, false><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, false><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else k_apply_tiled<R, MODE, EPI, 1, false><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
         }
+        if (after_main) hipEventRecord(after_main, s);
         launch_finish<EPI>(s, nx * ny * nch, nrhs, part, scal, rn, mask);
         return;
     }
@@ -794,7 +806,19 @@ static void launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int
         if (op->has_mu) k_apply<R, MODE, false, true, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, scal, mask);
         else k_apply<R, MODE, false, false, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, scal, mask);
     }
+    if (after_main) hipEventRecord(after_main, s);
     launch_finish<EPI>(s, (int)grid.x, nrhs, part, scal, rn, mask);
+}
+
+template <typename R>
+static void launch_upd_r(dim3 gv, hipStream_t s, cx<R> *r, const cx<R> *t, int64_t n, double *part, const Scal *scal) {
+    k_upd_r<R><<<gv, BS, 0, s>>>(r, t, n, n, part, scal);
+}
+
+template <typename R>
+static void launch_upd_dzq(dim3 gv, hipStream_t s, cx<R> *d, cx<R> *z, cx<R> *q, const cx<R> *r, int64_t n,
+                           const Scal *scal) {
+    k_upd_dzq<R><<<gv, BS, 0, s>>>(d, z, q, r, n, n, scal);
 }
 
 template <typename R>
@@ -805,9 +829,9 @@ static void launch_iter(mfd_op *op, int64_t nrhs) {
     dim3 gv(ngrid(n), (unsigned)nrhs);
     size_t pslot = (size_t)nrhs * NPART * 4;
     launch_apply<R, MODE_P, EPI_DOT>(op, (const cx<R> *)w.q, (cx<R> *)w.t, nullptr, nrhs, w.part, w.scal, nullptr);
-    k_upd_r<R><<<gv, BS, 0, s>>>((cx<R> *)w.r, (const cx<R> *)w.t, n, n, w.part + pslot, w.scal);
+    launch_upd_r<R>(gv, s, (cx<R> *)w.r, (const cx<R> *)w.t, n, w.part + pslot, w.scal);
     k_fin_r<<<(unsigned)nrhs, BS, 0, s>>>(w.part + pslot, (int)gv.x, w.scal);
-    k_upd_dzq<R><<<gv, BS, 0, s>>>((cx<R> *)w.d, (cx<R> *)w.z, (cx<R> *)w.q, (const cx<R> *)w.r, n, n, w.scal);
+    launch_upd_dzq<R>(gv, s, (cx<R> *)w.d, (cx<R> *)w.z, (cx<R> *)w.q, (const cx<R> *)w.r, n, w.scal);
 }
 
 }  // namespace mfd
@@ -879,6 +903,10 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
     op->d64.q = q;
     op->d64.om2 = om2;
     op->d64.wnode = wn;
+    // with a complex Bloch phase e^{-ikL} the operator is not complex-symmetric
+    // (the wrap entries of Cₑ and Cₘ are conjugate, not transposed, images)
+    for (int a = 0; a < 3; ++a)
+        if (op->G.d64.bloch[a] && fabs(op->G.d64.ph[a].im) > 1e-14) op->sym = 0;
     op->gd_ok = !full && !op->has_mu && !(om2.re == 0.0 && om2.im == 0.0);
     op->gd = 0;   // opt-in: it helps small PML-dominated grids and slows larger ones
     const char *wm = getenv("MFD_GD_WMODE");
@@ -1008,11 +1036,12 @@ __global__ void k_bench_scal(Scal *scal, int64_t nrhs) {
 }
 
 // Average duration of each kernel of one SQMR iteration (stencil with the
-// qᵀPq epilogue, r update with its reductions, d/z/q update), each timed over
-// `iters` back-to-back launches on the context stream with the solver's own
-// workspace and launch shapes.
+// qᵀPq epilogue, r update, d/z/q update), timed in context: `iters` complete
+// iterations (finish kernels included) run back to back on the context stream
+// with the solver's workspace and launch shapes, with events around the three
+// main kernels only.
 int op_bench_iteration_impl(mfd_op *op, int64_t nrhs, int prec, int iters, double *ms) {
-    if (nrhs < 1 || iters < 1) { set_error("nrhs and iters must be positive"); return MFD_ERR_ARG; }
+    if (nrhs < 1 || iters < 1 || iters > 1000) { set_error("nrhs must be positive and iters in [1, 1000]"); return MFD_ERR_ARG; }
     prec = prec == MFD_MIXED ? MFD_MIXED : MFD_FP64;
     MFD_TRY(ws_ensure(op, nrhs, prec));
     Workspace &w = op->ws;
@@ -1022,74 +1051,89 @@ int op_bench_iteration_impl(mfd_op *op, int64_t nrhs, int prec, int iters, doubl
     // bounded nonzero contents (every byte 0x3c: ~1e-2 in FP32, ~2e-19 in FP64)
     for (void *v : {w.r, w.q, w.t, w.d, w.z}) MFD_HIP(hipMemsetAsync(v, 0x3c, es * n * nrhs, s));
     dim3 gv(ngrid(n), (unsigned)nrhs);
-    unsigned gs = (unsigned)((nrhs + 63) / 64);
-    auto timed = [&](int which) -> double {
-        auto one = [&]() {
-            if (which == 0) {
-                if (prec == MFD_FP64)
-                    launch_apply<double, MODE_P, EPI_DOT>(op, (const c64 *)w.q, (c64 *)w.t, nullptr, nrhs, w.part, w.scal, nullptr);
-                else launch_apply<float, MODE_P, EPI_DOT>(op, (const c32 *)w.q, (c32 *)w.t, nullptr, nrhs, w.part, w.scal, nullptr);
-            } else if (which == 1) {
-                size_t ps = (size_t)nrhs * NPART * 4;
-                if (prec == MFD_FP64) k_upd_r<double><<<gv, BS, 0, s>>>((c64 *)w.r, (const c64 *)w.t, n, n, w.part + ps, w.scal);
-                else k_upd_r<float><<<gv, BS, 0, s>>>((c32 *)w.r, (const c32 *)w.t, n, n, w.part + ps, w.scal);
-                k_fin_r<<<(unsigned)nrhs, BS, 0, s>>>(w.part + ps, (int)gv.x, w.scal);
-            } else {
-                if (prec == MFD_FP64) k_upd_dzq<double><<<gv, BS, 0, s>>>((c64 *)w.d, (c64 *)w.z, (c64 *)w.q, (const c64 *)w.r, n, n, w.scal);
-                else k_upd_dzq<float><<<gv, BS, 0, s>>>((c32 *)w.d, (c32 *)w.z, (c32 *)w.q, (const c32 *)w.r, n, n, w.scal);
-            }
-        };
-        k_bench_scal<<<gs, 64, 0, s>>>(w.scal, nrhs);
-        for (int k = 0; k < 3; ++k) one();
-        k_bench_scal<<<gs, 64, 0, s>>>(w.scal, nrhs);
-        hipEventRecord(op->ev0, s);
-        for (int it = 0; it < iters; ++it) one();
-        hipEventRecord(op->ev1, s);
-        if (hipEventSynchronize(op->ev1) != hipSuccess) return -1.0;
-        float t = 0;
-        hipEventElapsedTime(&t, op->ev0, op->ev1);
-        return t / iters;
+    const size_t ps = (size_t)nrhs * NPART * 4;
+    std::vector<hipEvent_t> ev(6 * (size_t)iters, nullptr);
+    int rc = MFD_OK;
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) { rc = MFD_ERR_HIP; break; }
+    auto iteration = [&](hipEvent_t *E) {
+        if (E) hipEventRecord(E[0], s);
+        if (prec == MFD_FP64)
+            launch_apply<double, MODE_P, EPI_DOT>(op, (const c64 *)w.q, (c64 *)w.t, nullptr, nrhs, w.part, w.scal, nullptr, nullptr, E ? E[1] : nullptr);
+        else launch_apply<float, MODE_P, EPI_DOT>(op, (const c32 *)w.q, (c32 *)w.t, nullptr, nrhs, w.part, w.scal, nullptr, nullptr, E ? E[1] : nullptr);
+        if (E) hipEventRecord(E[2], s);
+        if (prec == MFD_FP64) launch_upd_r<double>(gv, s, (c64 *)w.r, (const c64 *)w.t, n, w.part + ps, w.scal);
+        else launch_upd_r<float>(gv, s, (c32 *)w.r, (const c32 *)w.t, n, w.part + ps, w.scal);
+        if (E) hipEventRecord(E[3], s);
+        k_fin_r<<<(unsigned)nrhs, BS, 0, s>>>(w.part + ps, (int)gv.x, w.scal);
+        if (E) hipEventRecord(E[4], s);
+        if (prec == MFD_FP64) launch_upd_dzq<double>(gv, s, (c64 *)w.d, (c64 *)w.z, (c64 *)w.q, (const c64 *)w.r, n, w.scal);
+        else launch_upd_dzq<float>(gv, s, (c32 *)w.d, (c32 *)w.z, (c32 *)w.q, (const c32 *)w.r, n, w.scal);
+        if (E) hipEventRecord(E[5], s);
     };
-    for (int k = 0; k < 3; ++k) {
-        ms[k] = timed(k);
-        if (ms[k] < 0) { set_error("kernel timing failed"); return MFD_ERR_HIP; }
+    if (!rc) {
+        k_bench_scal<<<(unsigned)((nrhs + 63) / 64), 64, 0, s>>>(w.scal, nrhs);
+        for (int k = 0; k < 3; ++k) iteration(nullptr);
+        for (int it = 0; it < iters; ++it) iteration(&ev[6 * (size_t)it]);
+        if (hipStreamSynchronize(s) != hipSuccess) rc = MFD_ERR_HIP;
     }
+    if (!rc) {
+        double acc[3] = {0, 0, 0};
+        for (int it = 0; it < iters; ++it)
+            for (int k = 0; k < 3; ++k) {
+                float t = 0;
+                hipEventElapsedTime(&t, ev[6 * (size_t)it + 2 * k], ev[6 * (size_t)it + 2 * k + 1]);
+                acc[k] += t;
+            }
+        for (int k = 0; k < 3; ++k) ms[k] = acc[k] / iters;
+    }
+    for (auto &e : ev)
+        if (e) hipEventDestroy(e);
+    if (rc) { set_error("kernel timing failed"); return rc; }
     MFD_HIP(hipGetLastError());
     return MFD_OK;
 }
 
 // Tile-shape autotuner: times the Krylov iteration's stencil (the Jacobi-scaled
-// operator P on FP64 or complex-FP32 vectors) for each candidate (TY, KZ) and
-// keeps the fastest per precision.
+// operator P on FP64 or complex-FP32 vectors) for each candidate (TY, KZ, tile
+// order) and keeps the fastest per precision.
 int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best32, double *ms_out) {
     if (op->full || op->has_mu) {   // the direct kernel serves these operators
         op->tile64[0] = op->tile32[0] = 0;
-        if (best64) best64[0] = best64[1] = 0;
-        if (best32) best32[0] = best32[1] = 0;
+        if (best64) best64[0] = best64[1] = best64[2] = 0;
+        if (best32) best32[0] = best32[1] = best32[2] = 0;
         return MFD_OK;
     }
-    const int tys[3] = {1, 2, 4}, kzs[4] = {4, 8, 16, 32};
+    const int tys[4] = {1, 2, 4, 8}, kzs[4] = {4, 8, 16, 32};
     double bestms[2] = {1e30, 1e30};
-    int bt[2][2] = {{4, 16}, {4, 16}};
+    int bt[2][3] = {{4, 16, 1}, {4, 16, 1}};
     for (int prec = 0; prec < 2; ++prec)
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 4; ++b) {
-                int *tile = prec == 0 ? op->tile64 : op->tile32;
-                tile[0] = tys[a];
-                tile[1] = kzs[b];
-                double ms = 0;
-                MFD_TRY(time_apply(op, nrhs, prec == 0 ? MFD_FP64 : MFD_MIXED, MODE_P, iters, &ms));
-                if (ms < bestms[prec]) { bestms[prec] = ms; bt[prec][0] = tys[a]; bt[prec][1] = kzs[b]; }
-            }
-    op->tile64[0] = bt[0][0]; op->tile64[1] = bt[0][1];
-    op->tile32[0] = bt[1][0]; op->tile32[1] = bt[1][1];
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b)
+                for (int xo = 0; xo < 2; ++xo) {
+                    int *tile = prec == 0 ? op->tile64 : op->tile32;
+                    tile[0] = tys[a];
+                    tile[1] = kzs[b];
+                    tile[2] = xo;
+                    double ms = 0;
+                    MFD_TRY(time_apply(op, nrhs, prec == 0 ? MFD_FP64 : MFD_MIXED, MODE_P, iters, &ms));
+                    if (ms < bestms[prec]) {
+                        bestms[prec] = ms;
+                        bt[prec][0] = tys[a];
+                        bt[prec][1] = kzs[b];
+                        bt[prec][2] = xo;
+                    }
+                }
+    for (int k = 0; k < 3; ++k) {
+        op->tile64[k] = bt[0][k];
+        op->tile32[k] = bt[1][k];
+        if (best64) best64[k] = bt[0][k];
+        if (best32) best32[k] = bt[1][k];
+    }
     op->ws.release();   // captured graphs hold the old launch shapes
-    if (best64) { best64[0] = bt[0][0]; best64[1] = bt[0][1]; }
-    if (best32) { best32[0] = bt[1][0]; best32[1] = bt[1][1]; }
     if (ms_out) { ms_out[0] = bestms[0]; ms_out[1] = bestms[1]; }
     return MFD_OK;
 }
-
 
 // ------------------------------------------------ COCG with reliable updates
 //
@@ -1102,20 +1146,20 @@ int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best
 // the search direction p is kept.  Convergence is decided on that FP64
 // residual only.
 
-// x += α p;  r −= α t;  partial sums of ||r||², rᵀr (k_fin_cocg finishes)
+// r −= α t;  partial sums of ||r||², rᵀr (k_fin_cocg finishes).  The iterate
+// update x += α p of this step is deferred to the next step's stencil, which
+// reads p anyway.
 template <typename R>
-__global__ __launch_bounds__(BS) void k_cocg_xr(cx<R> *__restrict__ xall, cx<R> *__restrict__ rall,
-                                                const cx<R> *__restrict__ pall, const cx<R> *__restrict__ tall,
-                                                int64_t n, int64_t vs, double *part, const Scal *scal) {
+__global__ __launch_bounds__(BS) void k_cocg_r(cx<R> *__restrict__ rall, const cx<R> *__restrict__ tall, int64_t n,
+                                               int64_t vs, double *part, const Scal *scal) {
     const int rhs = blockIdx.y;
     const Scal &S = scal[rhs];
     if (S.done || S.breakdown) return;
-    cx<R> *x = xall + rhs * vs, *r = rall + rhs * vs;
-    const cx<R> *p = pall + rhs * vs, *t = tall + rhs * vs;
+    cx<R> *r = rall + rhs * vs;
+    const cx<R> *t = tall + rhs * vs;
     c64 al = mk<double>(S.alpha_re, S.alpha_im);
     double acc[3] = {0, 0, 0};
     for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
-        x[e] = cvt<R>(cvt<double>(x[e]) + al * cvt<double>(p[e]));
         c64 v = cvt<double>(r[e]) - al * cvt<double>(t[e]);
         r[e] = cvt<R>(v);
         acc[0] += (double)v.re * v.re + (double)v.im * v.im;
@@ -1125,7 +1169,7 @@ __global__ __launch_bounds__(BS) void k_cocg_xr(cx<R> *__restrict__ xall, cx<R> 
     block_partial<3>(acc, part + (size_t)rhs * NPART * 4, (int)blockIdx.x);
 }
 
-// after k_cocg_xr: β, ρ, running max of ||r|| (one block per RHS)
+// after k_cocg_r: β, ρ, running max of ||r|| (one block per RHS)
 __global__ __launch_bounds__(BS) void k_fin_cocg(const double *part, int nb, Scal *scal) {
     const int rhs = blockIdx.x;
     Scal &S = scal[rhs];
@@ -1148,30 +1192,38 @@ __global__ __launch_bounds__(BS) void k_fin_cocg(const double *part, int nb, Sca
     S.it += 1;
 }
 
-// p = r + β p
+// p = r + β p_old and the deferred x += α_prev p_old, for operators served by
+// the direct stencil (the tiled stencil does this while staging)
 template <typename R>
-__global__ __launch_bounds__(BS) void k_cocg_p(cx<R> *__restrict__ pall, const cx<R> *__restrict__ rall, int64_t n,
+__global__ __launch_bounds__(BS) void k_cocg_p(cx<R> *__restrict__ pnall, const cx<R> *__restrict__ poall,
+                                               const cx<R> *__restrict__ rall, cx<R> *__restrict__ xall, int64_t n,
                                                int64_t vs, const Scal *scal) {
     const int rhs = blockIdx.y;
     const Scal &S = scal[rhs];
     if (S.done || S.breakdown) return;
-    cx<R> *p = pall + rhs * vs;
-    const cx<R> *r = rall + rhs * vs;
-    c64 be = mk<double>(S.beta_re, S.beta_im);
-    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS)
-        p[e] = cvt<R>(cvt<double>(r[e]) + be * cvt<double>(p[e]));
+    cx<R> *pn = pnall + rhs * vs, *x = xall + rhs * vs;
+    const cx<R> *po = poall + rhs * vs, *r = rall + rhs * vs;
+    c64 be = mk<double>(S.beta_re, S.beta_im), ap = mk<double>(S.apend_re, S.apend_im);
+    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+        c64 pv = cvt<double>(po[e]);
+        pn[e] = cvt<R>(cvt<double>(r[e]) + be * pv);
+        x[e] = cvt<R>(cvt<double>(x[e]) + ap * pv);
+    }
 }
 
-// e64 += (Q T⁻¹) x;  x = 0   (masked RHS)
+// e64 += (Q T⁻¹)(x + α_pend p);  x = 0   (masked RHS; k_ru_replace then clears α_pend)
 template <typename R>
 __global__ __launch_bounds__(BS) void k_ru_flush(DGrid<double> g, OpData<double> od, c64 *__restrict__ eall,
-                                                 cx<R> *__restrict__ xall, int64_t n, int64_t vs, const int *mask) {
+                                                 cx<R> *__restrict__ xall, const cx<R> *__restrict__ pall, int64_t n,
+                                                 int64_t vs, const int *mask, const Scal *scal) {
     const int rhs = blockIdx.y;
     if (!mask[rhs]) return;
     c64 *e64 = eall + rhs * vs;
     cx<R> *x = xall + rhs * vs;
+    const cx<R> *pv = pall + rhs * vs;
+    const c64 ap = mk<double>(scal[rhs].apend_re, scal[rhs].apend_im);
     for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
-        cx<R> xe = x[e];
+        c64 xe = cvt<double>(x[e]) + ap * cvt<double>(pv[e]);
         if (xe.re == 0 && xe.im == 0) continue;
         int c;
         int64_t p;
@@ -1179,7 +1231,7 @@ __global__ __launch_bounds__(BS) void k_ru_flush(DGrid<double> g, OpData<double>
         int i, j, k;
         cell3(g, p, i, j, k);
         c64 rf = cvt<double>(od.q[3 * p + c]) * tfac(g, c, i, j, k, true);
-        e64[e] = e64[e] + rf * cvt<double>(xe);
+        e64[e] = e64[e] + rf * xe;
         x[e] = czero<R>();
     }
 }
@@ -1223,6 +1275,8 @@ __global__ __launch_bounds__(BS) void k_ru_replace(DGrid<double> g, OpData<doubl
     S.tau_ref = S.cn;          // running maximum restarts here
     S.tau = sqrt(rn_true[rhs]);   // its true (unscaled) norm
     S.breakdown = 0;
+    S.apend_re = S.apend_im = 0;  // the flush took the pending update
+    if (restart[rhs]) S.beta_re = S.beta_im = 0;   // next p = r
     if (S.cn == 0.0) S.done = 1;
 }
 
@@ -1236,25 +1290,53 @@ __global__ void k_scal_reset(Scal *scal, int64_t nrhs) {
     if (r < nrhs) memset(&scal[r], 0, sizeof(Scal));
 }
 
+// COCG p buffers: step parity k reads p_old from buffer k and writes p to k^1
 template <typename R>
-static void launch_cocg_iter(mfd_op *op, int64_t nrhs) {
+static inline cx<R> *cocg_p(Workspace &w, int k) { return (cx<R> *)(k ? w.d : w.q); }
+
+// one COCG step of parity par: p = r + β p_old (formed in the stencil), t = P p,
+// σ = pᵀt, α; the deferred x += α_prev p_old; r −= α t; β, ρ
+template <typename R>
+static void launch_cocg_iter(mfd_op *op, int64_t nrhs, int par) {
     Workspace &w = op->ws;
     int64_t n = w.n;
     hipStream_t s = op->stream;
     dim3 gv(ngrid(n), (unsigned)nrhs);
     size_t pslot = (size_t)nrhs * NPART * 4;
-    // w.q holds p, w.t holds t = P p, w.z holds x, w.r holds r
-    launch_apply<R, MODE_P, EPI_DOT>(op, (const cx<R> *)w.q, (cx<R> *)w.t, nullptr, nrhs, w.part, w.scal, nullptr);
-    k_cocg_xr<R><<<gv, BS, 0, s>>>((cx<R> *)w.z, (cx<R> *)w.r, (const cx<R> *)w.q, (const cx<R> *)w.t, n, n,
-                                   w.part + pslot, w.scal);
+    cx<R> *po = cocg_p<R>(w, par), *pn = cocg_p<R>(w, par ^ 1);
+    const DGrid<R> &g = op->G.view<R>();
+    int ty = std::is_same<R, double>::value ? op->tile64[0] : op->tile32[0];
+    if (!op->full && !op->has_mu && !op->gd && ty > 0) {
+        int kz = std::is_same<R, double>::value ? op->tile64[1] : op->tile32[1];
+        int xr = std::is_same<R, double>::value ? op->tile64[2] : op->tile32[2];
+        int nx = (g.N[0] + 63) / 64, ny = (g.N[1] + ty - 1) / ty, nch = (g.N[2] + kz - 1) / kz;
+        while ((int64_t)nx * ny * nch > NPART) { kz *= 2; nch = (g.N[2] + kz - 1) / kz; }
+        dim3 grid(nx, ny, (unsigned)(nch * nrhs));
+        int64_t vs = 3 * g.M;
+        const cx<R> *rr = (const cx<R> *)w.r;
+        cx<R> *xi = (cx<R> *)w.z, *t = (cx<R> *)w.t;
+        if (ty == 8)
+            k_apply_tiled<R, MODE_P, EPI_DOT, 8, false, true><<<grid, 512, 0, s>>>(op->G.d64, op->d64, op->d64.om2, po, t, nullptr, vs, kz, nch, w.part, w.scal, nullptr, xr, rr, pn, xi);
+        else if (ty == 2)
+            k_apply_tiled<R, MODE_P, EPI_DOT, 2, false, true><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, po, t, nullptr, vs, kz, nch, w.part, w.scal, nullptr, xr, rr, pn, xi);
+        else if (ty == 4)
+            k_apply_tiled<R, MODE_P, EPI_DOT, 4, false, true><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, po, t, nullptr, vs, kz, nch, w.part, w.scal, nullptr, xr, rr, pn, xi);
+        else
+            k_apply_tiled<R, MODE_P, EPI_DOT, 1, false, true><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, po, t, nullptr, vs, kz, nch, w.part, w.scal, nullptr, xr, rr, pn, xi);
+        launch_finish<EPI_DOT>(s, nx * ny * nch, nrhs, w.part, w.scal, nullptr, nullptr);
+    } else {
+        k_cocg_p<R><<<gv, BS, 0, s>>>(pn, po, (const cx<R> *)w.r, (cx<R> *)w.z, n, n, w.scal);
+        launch_apply<R, MODE_P, EPI_DOT>(op, pn, (cx<R> *)w.t, nullptr, nrhs, w.part, w.scal, nullptr);
+    }
+    k_cocg_r<R><<<gv, BS, 0, s>>>((cx<R> *)w.r, (const cx<R> *)w.t, n, n, w.part + pslot, w.scal);
     k_fin_cocg<<<(unsigned)nrhs, BS, 0, s>>>(w.part + pslot, (int)gv.x, w.scal);
-    k_cocg_p<R><<<gv, BS, 0, s>>>((cx<R> *)w.q, (const cx<R> *)w.r, n, n, w.scal);
 }
 
 // COCG with reliable updates; b, e device FP64; returns iterations and updates
 template <typename R>
 static int run_cocg(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_solve_opts &o, double delta,
-                    long long *iters_out, long long *updates_out, double *worst_out, bool *conv_out) {
+                    long long *iters_out, long long *updates_out, double *worst_out, bool *conv_out,
+                    bool bail_on_stall) {
     Workspace &w = op->ws;
     hipStream_t s = op->stream;
     int64_t n = w.n;
@@ -1267,6 +1349,9 @@ static int run_cocg(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_
     MFD_HIP(hipMemcpyAsync(bn.data(), w.bn, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
     k_scal_reset<<<(unsigned)((nrhs + 63) / 64), 64, 0, s>>>(w.scal, nrhs);
     MFD_HIP(hipMemsetAsync(w.z, 0, sizeof(cx<R>) * n * nrhs, s));
+    MFD_HIP(hipMemsetAsync(w.q, 0, sizeof(cx<R>) * n * nrhs, s));
+    MFD_HIP(hipMemsetAsync(w.d, 0, sizeof(cx<R>) * n * nrhs, s));
+    int par = 0;   // parity of the next step (which p buffer holds the current p)
     std::vector<int> mask(nrhs, 1), restart(nrhs, 1), done(nrhs, 0);
     std::vector<double> relres(nrhs, 1.0);
     std::vector<long long> last_ru(nrhs, 0);
@@ -1299,7 +1384,7 @@ static int run_cocg(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_
         for (int64_t r = 0; r < nrhs; ++r) m2[r] = mask[r] && !done[r];
         MFD_HIP(hipMemcpyAsync(mask_d, m2.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s));
         MFD_HIP(hipMemcpyAsync(aux_d, restart.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s));
-        k_ru_replace<R><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (const c64 *)w.r64, (cx<R> *)w.r, (cx<R> *)w.q, n, n,
+        k_ru_replace<R><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (const c64 *)w.r64, (cx<R> *)w.r, cocg_p<R>(w, par), n, n,
                                           w.part + 2 * pslot, w.cnt + 2 * nrhs, w.scal, mask_d, aux_d, w.rn);
         MFD_HIP(hipMemcpyAsync(aux_d, done.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s));
         k_set_done<<<(unsigned)((nrhs + 63) / 64), 64, 0, s>>>(w.scal, aux_d, nrhs);
@@ -1310,27 +1395,32 @@ static int run_cocg(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_
     MFD_TRY(residual(&worst));
     bool all = true;
     for (int64_t r = 0; r < nrhs; ++r) all = all && done[r];
-    long long updates = 0, mx = 0;
+    long long updates = 0, mx = 0, best_upd = 0;
+    double best_worst = worst;
+    bool stalled = false;
     if (!all) {
         MFD_TRY(replace());
         std::fill(restart.begin(), restart.end(), 0);
         int chunk = o.check_every > 0 ? o.check_every : 32;
         bool use_graph = o.use_graph != 0;
-        if (use_graph && (w.graph_iters != chunk || w.graph_kind != 2)) {
+        if (use_graph && (chunk & 1)) ++chunk;   // a replayed graph must end on the parity it starts with
+        const int kind = sizeof(R) == 4 ? 2 : 5;
+        if (use_graph && (w.graph_iters != chunk || w.graph_kind != kind)) {
             if (w.graph) { hipGraphExecDestroy(w.graph); w.graph = nullptr; }
             hipGraph_t gr;
             MFD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            for (int it = 0; it < chunk; ++it) launch_cocg_iter<R>(op, nrhs);
+            for (int it = 0; it < chunk; ++it) launch_cocg_iter<R>(op, nrhs, (par + it) & 1);
             MFD_HIP(hipStreamEndCapture(s, &gr));
             MFD_HIP(hipGraphInstantiate(&w.graph, gr, nullptr, nullptr, 0));
             hipGraphDestroy(gr);
             w.graph_iters = chunk;
-            w.graph_kind = 2;
+            w.graph_kind = kind;
         }
         while (true) {
             if (use_graph) MFD_HIP(hipGraphLaunch(w.graph, s));
             else
-                for (int it = 0; it < chunk; ++it) launch_cocg_iter<R>(op, nrhs);
+                for (int it = 0; it < chunk; ++it) launch_cocg_iter<R>(op, nrhs, (par + it) & 1);
+            par = (par + chunk) & 1;
             MFD_HIP(hipGetLastError());
             MFD_HIP(hipMemcpyAsync(w.scal_h, w.scal, sizeof(Scal) * nrhs, hipMemcpyDeviceToHost, s));
             MFD_HIP(hipStreamSynchronize(s));
@@ -1353,12 +1443,20 @@ static int run_cocg(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_
                 if (trig) { mask[r] = 1; restart[r] = S.breakdown || rn > 1e4 * S.cn; any = true; last_ru[r] = S.it; }
             }
             if (any) {
-                k_ru_flush<R><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (c64 *)e, (cx<R> *)w.z, n, n, up_mask());
+                k_ru_flush<R><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (c64 *)e, (cx<R> *)w.z, cocg_p<R>(w, par), n, n, up_mask(), w.scal);
                 MFD_TRY(residual(&worst));
+                // a stalling low-precision run hands over (the flush above put
+                // everything it produced into e): three updates without a 2x
+                // gain on the best true residual, or a 10x rise above it
+                if (bail_on_stall) {
+                    if (worst < 0.5 * best_worst) { best_worst = worst; best_upd = updates; }
+                    else if (worst > 10 * best_worst || updates - best_upd >= 3) stalled = true;
+                }
                 MFD_TRY(replace());
                 ++updates;
                 if (getenv("MFD_SOLVE_TRACE"))
-                    fprintf(stderr, "[mfd cocg] it %lld update %lld worst residual %.3e\\n", mx, updates, worst);
+                    fprintf(stderr, "[mfd cocg] it %lld update %lld worst residual %.3e\n", mx, updates, worst);
+                if (stalled) break;
             }
             all = true;
             for (int64_t r = 0; r < nrhs; ++r) all = all && done[r];
@@ -1366,7 +1464,7 @@ static int run_cocg(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_
         }
         if (!all) {   // flush what the last iterations produced
             std::fill(mask.begin(), mask.end(), 1);
-            k_ru_flush<R><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (c64 *)e, (cx<R> *)w.z, n, n, up_mask());
+            k_ru_flush<R><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (c64 *)e, (cx<R> *)w.z, cocg_p<R>(w, par), n, n, up_mask(), w.scal);
             MFD_TRY(residual(&worst));
         }
     }
@@ -1379,11 +1477,15 @@ static int run_cocg(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_
     return MFD_OK;
 }
 
-int op_set_tile_impl(mfd_op *op, int prec, int ty, int kz) {
-    if (!(ty == 0 || ty == 1 || ty == 2 || ty == 4) || kz < 1) { set_error("tile: TY in {0,1,2,4}, KZ >= 1"); return MFD_ERR_ARG; }
+int op_set_tile_impl(mfd_op *op, int prec, int ty, int kz, int xcd) {
+    if (!(ty == 0 || ty == 1 || ty == 2 || ty == 4 || ty == 8) || kz < 1 || (xcd != 0 && xcd != 1)) {
+        set_error("tile: TY in {0,1,2,4,8}, KZ >= 1, order 0 or 1");
+        return MFD_ERR_ARG;
+    }
     int *t = prec == MFD_FP64 ? op->tile64 : op->tile32;
     t[0] = ty;
     t[1] = kz;
+    t[2] = xcd;
     op->ws.release();
     return MFD_OK;
 }
@@ -1475,6 +1577,237 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
     return MFD_OK;
 }
 
+// ------------------------------------------------ BiCGSTAB (non-symmetric P)
+//
+// van der Vorst's BiCGSTAB on the Jacobi-scaled P, FP64, for operators that
+// are not complex-symmetric (Bloch phases with k ≠ 0), where SQMR and COCG do
+// not apply.  Vectors: r (w.r), r̂ (w.et), p (w.q), v (w.t), s (w.d),
+// t (w.et2), x (w.z).  Hermitian inner products; the recurrence residual ‖r‖
+// is compared with the cycle target as in SQMR.
+
+// cycle start: r = r̂ = p = (Q T) r64, x = v = 0; ρ = r̂ᴴr, τ = ‖r‖
+__global__ __launch_bounds__(BS) void k_bi_init(DGrid<double> g, OpData<double> od, const c64 *__restrict__ r64all,
+                                                c64 *rall, c64 *rhall, c64 *pall, c64 *vall, c64 *xall, int64_t n,
+                                                int64_t vs, double *part, unsigned *cnt, Scal *scal,
+                                                double rel_target, const int *active, long long itmax) {
+    const int rhs = blockIdx.y;
+    const c64 *r64 = r64all + rhs * vs;
+    c64 *r = rall + rhs * vs, *rh = rhall + rhs * vs, *pp = pall + rhs * vs, *v = vall + rhs * vs, *x = xall + rhs * vs;
+    bool act = active[rhs] != 0;
+    double acc[1] = {0};
+    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+        int c;
+        int64_t p;
+        dof2cp(g, e, c, p);
+        int i, j, k;
+        cell3(g, p, i, j, k);
+        c64 val = czero<double>();
+        if (act) val = (od.q[3 * p + c] * tfac(g, c, i, j, k, false)) * r64[e];
+        r[e] = val;
+        rh[e] = val;
+        pp[e] = val;
+        v[e] = czero<double>();
+        x[e] = czero<double>();
+        acc[0] += val.re * val.re + val.im * val.im;
+    }
+    double tot[1];
+    if (!reduce_last<1>(acc, part + (size_t)rhs * NPART * 4, cnt + rhs, tot)) return;
+    if (threadIdx.x != 0) return;
+    Scal &S = scal[rhs];
+    memset(&S, 0, sizeof(Scal));
+    S.tau = sqrt(tot[0]);
+    S.cn = S.tau;
+    S.rho_re = tot[0];
+    S.target = rel_target * S.cn;
+    S.itmax = itmax;
+    if (!act || S.cn == 0.0) { S.conv = 1; S.done = 1; }
+}
+
+// partials of aᴴb (and ‖a‖² when NV = 3)
+template <int NV>
+__global__ __launch_bounds__(BS) void k_bi_dot(const c64 *__restrict__ aall, const c64 *__restrict__ ball, int64_t n,
+                                               int64_t vs, double *part, const Scal *scal) {
+    const int rhs = blockIdx.y;
+    const Scal &S = scal[rhs];
+    if (S.done || S.conv) return;
+    const c64 *a = aall + rhs * vs, *b = ball + rhs * vs;
+    double acc[NV];
+    for (int k = 0; k < NV; ++k) acc[k] = 0;
+    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+        c64 x = a[e], y = b[e];
+        acc[0] += x.re * y.re + x.im * y.im;
+        acc[1] += x.re * y.im - x.im * y.re;
+        if (NV == 3) acc[2] += x.re * x.re + x.im * x.im;
+    }
+    block_partial<NV>(acc, part + (size_t)rhs * NPART * 4, (int)blockIdx.x);
+}
+
+// α = ρ / (r̂ᴴv)
+__global__ __launch_bounds__(BS) void k_bi_fin1(const double *part, int nb, Scal *scal) {
+    const int rhs = blockIdx.x;
+    Scal &S = scal[rhs];
+    if (S.done || S.conv) return;
+    double tot[2];
+    sum_partials<2>(part + (size_t)rhs * NPART * 4, nb, tot);
+    if (threadIdx.x != 0) return;
+    if (tot[0] == 0.0 && tot[1] == 0.0) { S.breakdown = 1; S.conv = 1; S.alpha_re = S.alpha_im = 0; return; }
+    c64 a = cdiv(mk<double>(S.rho_re, S.rho_im), mk<double>(tot[0], tot[1]));
+    S.alpha_re = a.re;
+    S.alpha_im = a.im;
+}
+
+// s = r − α v
+__global__ __launch_bounds__(BS) void k_bi_s(c64 *__restrict__ sall, const c64 *__restrict__ rall,
+                                             const c64 *__restrict__ vall, int64_t n, int64_t vs, const Scal *scal) {
+    const int rhs = blockIdx.y;
+    const Scal &S = scal[rhs];
+    if (S.done || S.conv) return;
+    c64 *s = sall + rhs * vs;
+    const c64 *r = rall + rhs * vs, *v = vall + rhs * vs;
+    c64 al = mk<double>(S.alpha_re, S.alpha_im);
+    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) s[e] = r[e] - al * v[e];
+}
+
+// ω = tᴴs / tᴴt
+__global__ __launch_bounds__(BS) void k_bi_fin2(const double *part, int nb, Scal *scal) {
+    const int rhs = blockIdx.x;
+    Scal &S = scal[rhs];
+    if (S.done || S.conv) return;
+    double tot[3];
+    sum_partials<3>(part + (size_t)rhs * NPART * 4, nb, tot);
+    if (threadIdx.x != 0) return;
+    if (tot[2] == 0.0) { S.om_re = S.om_im = 0; return; }   // t = 0: s is the residual, ω unused
+    S.om_re = tot[0] / tot[2];
+    S.om_im = tot[1] / tot[2];
+}
+
+// x += α p + ω s;  r = s − ω t;  partials of r̂ᴴr and ‖r‖²
+__global__ __launch_bounds__(BS) void k_bi_xr(c64 *__restrict__ xall, c64 *__restrict__ rall,
+                                              const c64 *__restrict__ pall, const c64 *__restrict__ sall,
+                                              const c64 *__restrict__ tall, const c64 *__restrict__ rhall, int64_t n,
+                                              int64_t vs, double *part, const Scal *scal) {
+    const int rhs = blockIdx.y;
+    const Scal &S = scal[rhs];
+    if (S.done || S.conv) return;
+    c64 *x = xall + rhs * vs, *r = rall + rhs * vs;
+    const c64 *pp = pall + rhs * vs, *s = sall + rhs * vs, *t = tall + rhs * vs, *rh = rhall + rhs * vs;
+    c64 al = mk<double>(S.alpha_re, S.alpha_im), om = mk<double>(S.om_re, S.om_im);
+    double acc[3] = {0, 0, 0};
+    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+        c64 se = s[e];
+        x[e] = x[e] + al * pp[e] + om * se;
+        c64 rv = se - om * t[e];
+        r[e] = rv;
+        c64 h = rh[e];
+        acc[0] += h.re * rv.re + h.im * rv.im;
+        acc[1] += h.re * rv.im - h.im * rv.re;
+        acc[2] += rv.re * rv.re + rv.im * rv.im;
+    }
+    block_partial<3>(acc, part + (size_t)rhs * NPART * 4, (int)blockIdx.x);
+}
+
+// β = (ρ_new/ρ)(α/ω), ρ, τ = ‖r‖, convergence
+__global__ __launch_bounds__(BS) void k_bi_fin3(const double *part, int nb, Scal *scal) {
+    const int rhs = blockIdx.x;
+    Scal &S = scal[rhs];
+    if (S.done || S.conv) return;
+    double tot[3];
+    sum_partials<3>(part + (size_t)rhs * NPART * 4, nb, tot);
+    if (threadIdx.x != 0) return;
+    c64 rho1 = mk<double>(tot[0], tot[1]), om = mk<double>(S.om_re, S.om_im);
+    S.tau = sqrt(tot[2]);
+    S.it += 1;
+    if ((S.rho_re == 0.0 && S.rho_im == 0.0) || (om.re == 0.0 && om.im == 0.0)) {
+        S.breakdown = 1;
+        S.conv = 1;
+        S.beta_re = S.beta_im = 0;
+    } else {
+        c64 b = cdiv(rho1, mk<double>(S.rho_re, S.rho_im)) * cdiv(mk<double>(S.alpha_re, S.alpha_im), om);
+        S.beta_re = b.re;
+        S.beta_im = b.im;
+    }
+    S.rho_re = rho1.re;
+    S.rho_im = rho1.im;
+    if (S.tau <= S.target || S.it >= S.itmax || S.tau == 0.0) S.conv = 1;
+}
+
+// p = r + β (p − ω v)
+__global__ __launch_bounds__(BS) void k_bi_p(c64 *__restrict__ pall, const c64 *__restrict__ rall,
+                                             const c64 *__restrict__ vall, int64_t n, int64_t vs, const Scal *scal) {
+    const int rhs = blockIdx.y;
+    const Scal &S = scal[rhs];
+    if (S.done || S.conv) return;
+    c64 *pp = pall + rhs * vs;
+    const c64 *r = rall + rhs * vs, *v = vall + rhs * vs;
+    c64 be = mk<double>(S.beta_re, S.beta_im), om = mk<double>(S.om_re, S.om_im);
+    for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS)
+        pp[e] = r[e] + be * (pp[e] - om * v[e]);
+}
+
+static void launch_bicgstab_iter(mfd_op *op, int64_t nrhs) {
+    Workspace &w = op->ws;
+    int64_t n = w.n;
+    hipStream_t s = op->stream;
+    dim3 gv(ngrid(n), (unsigned)nrhs);
+    const size_t ps = (size_t)nrhs * NPART * 4;
+    c64 *r = (c64 *)w.r, *rh = (c64 *)w.et, *pp = (c64 *)w.q, *v = (c64 *)w.t, *sv = (c64 *)w.d, *t = (c64 *)w.et2,
+        *x = (c64 *)w.z;
+    launch_apply<double, MODE_P, EPI_STORE>(op, pp, v, nullptr, nrhs, nullptr, nullptr, nullptr);
+    k_bi_dot<2><<<gv, BS, 0, s>>>(rh, v, n, n, w.part, w.scal);
+    k_bi_fin1<<<(unsigned)nrhs, BS, 0, s>>>(w.part, (int)gv.x, w.scal);
+    k_bi_s<<<gv, BS, 0, s>>>(sv, r, v, n, n, w.scal);
+    launch_apply<double, MODE_P, EPI_STORE>(op, sv, t, nullptr, nrhs, nullptr, nullptr, nullptr);
+    k_bi_dot<3><<<gv, BS, 0, s>>>(t, sv, n, n, w.part + ps, w.scal);
+    k_bi_fin2<<<(unsigned)nrhs, BS, 0, s>>>(w.part + ps, (int)gv.x, w.scal);
+    k_bi_xr<<<gv, BS, 0, s>>>(x, r, pp, sv, t, rh, n, n, w.part + 2 * ps, w.scal);
+    k_bi_fin3<<<(unsigned)nrhs, BS, 0, s>>>(w.part + 2 * ps, (int)gv.x, w.scal);
+    k_bi_p<<<gv, BS, 0, s>>>(pp, r, v, n, n, w.scal);
+}
+
+// one BiCGSTAB cycle from the FP64 residual in w.r64; x accumulates in w.z
+static int run_inner_bicgstab(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double rel_target,
+                              const int *active_d, long long budget, long long *iters_out) {
+    Workspace &w = op->ws;
+    hipStream_t s = op->stream;
+    int64_t n = w.n;
+    dim3 gv(ngrid(n), (unsigned)nrhs);
+    size_t pslot = (size_t)nrhs * NPART * 4;
+    k_bi_init<<<gv, BS, 0, s>>>(op->G.d64, op->d64, (const c64 *)w.r64, (c64 *)w.r, (c64 *)w.et, (c64 *)w.q,
+                                (c64 *)w.t, (c64 *)w.z, n, n, w.part + 2 * pslot, w.cnt + 2 * nrhs, w.scal, rel_target,
+                                active_d, budget);
+    MFD_HIP(hipGetLastError());
+    int chunk = o->check_every > 0 ? o->check_every : 32;
+    bool use_graph = o->use_graph != 0;
+    if (use_graph && (w.graph_iters != chunk || w.graph_kind != 4)) {
+        if (w.graph) { hipGraphExecDestroy(w.graph); w.graph = nullptr; }
+        hipGraph_t gr;
+        MFD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int it = 0; it < chunk; ++it) launch_bicgstab_iter(op, nrhs);
+        MFD_HIP(hipStreamEndCapture(s, &gr));
+        MFD_HIP(hipGraphInstantiate(&w.graph, gr, nullptr, nullptr, 0));
+        hipGraphDestroy(gr);
+        w.graph_iters = chunk;
+        w.graph_kind = 4;
+    }
+    while (true) {
+        if (use_graph) MFD_HIP(hipGraphLaunch(w.graph, s));
+        else
+            for (int it = 0; it < chunk; ++it) launch_bicgstab_iter(op, nrhs);
+        MFD_HIP(hipGetLastError());
+        MFD_HIP(hipMemcpyAsync(w.scal_h, w.scal, sizeof(Scal) * nrhs, hipMemcpyDeviceToHost, s));
+        MFD_HIP(hipStreamSynchronize(s));
+        bool all = true;
+        long long mx = 0;
+        for (int64_t r = 0; r < nrhs; ++r) {
+            if (!w.scal_h[r].conv) all = false;
+            if (w.scal_h[r].it > mx) mx = w.scal_h[r].it;
+        }
+        *iters_out = mx;
+        if (all || mx >= budget) break;
+    }
+    return MFD_OK;
+}
+
 // e (device, FP64, nrhs) solves A e = b (device) to o->tol.
 int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_solve_opts *oin, mfd_solve_stats *st) {
     mfd_solve_opts o;
@@ -1488,7 +1821,12 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
     size_t pslot = (size_t)nrhs * NPART * 4;
     dim3 gv(ngrid(n), (unsigned)nrhs);
     hipEventRecord(op->ev0, s);
-    int method = o.method >= 0 ? o.method : MFD_SQMR;
+    int method = o.method >= 0 ? o.method : (op->sym ? MFD_SQMR : MFD_BICGSTAB);
+    if (method != MFD_SQMR && method != MFD_COCG && method != MFD_BICGSTAB) { set_error("unknown method"); return MFD_ERR_ARG; }
+    if (!op->sym && method != MFD_BICGSTAB) {
+        set_error("SQMR and COCG need a complex-symmetric operator; with a complex Bloch phase use BiCGSTAB");
+        return MFD_ERR_UNSUPPORTED;
+    }
     if (method == MFD_COCG) {
         long long it = 0, upd = 0;
         double worst = 0;
@@ -1496,8 +1834,24 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
         // δ: FP32 recurrences are refreshed once the residual has fallen 100x;
         // FP64 ones only when convergence is near
         double delta = prec == MFD_MIXED ? 1e-2 : 0.0;
-        int rc = prec == MFD_MIXED ? run_cocg<float>(op, b, e, nrhs, o, delta, &it, &upd, &worst, &conv)
-                                   : run_cocg<double>(op, b, e, nrhs, o, delta, &it, &upd, &worst, &conv);
+        long long it32 = 0;
+        int rc;
+        if (prec == MFD_MIXED) {
+            // FP32 recurrences until the true residual stops improving, then FP64
+            // from the current iterate
+            rc = run_cocg<float>(op, b, e, nrhs, o, delta, &it, &upd, &worst, &conv, true);
+            if (rc) return rc;
+            it32 = it;
+            if (!conv && it < o.maxit) {
+                mfd_solve_opts o2 = o;
+                o2.maxit = o.maxit - it;
+                long long it2 = 0, upd2 = 0;
+                MFD_TRY(ws_ensure(op, nrhs, MFD_MIXED));
+                rc = run_cocg<double>(op, b, e, nrhs, o2, 0.0, &it2, &upd2, &worst, &conv, false);
+                it += it2;
+                upd += upd2;
+            }
+        } else rc = run_cocg<double>(op, b, e, nrhs, o, delta, &it, &upd, &worst, &conv, false);
         if (rc) return rc;
         hipEventRecord(op->ev1, s);
         hipEventSynchronize(op->ev1);
@@ -1509,7 +1863,7 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
             st->relres = worst;
             st->seconds = ms * 1e-3;
             st->converged = conv;
-            st->iterations_fp32 = prec == MFD_MIXED ? it : 0;
+            st->iterations_fp32 = it32;
         }
         if (!conv) { set_error("solve did not reach the tolerance within maxit"); return MFD_ERR_NOCONV; }
         return MFD_OK;
@@ -1524,7 +1878,7 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
     bool conv = false;
     // FP32 cycles run while each gains at least 10x on the FP64 residual; a
     // cycle that gains less switches the rest of the solve to FP64 cycles
-    int cyc_prec = prec;
+    int cyc_prec = method == MFD_BICGSTAB ? MFD_FP64 : prec;
     while (true) {
         launch_apply<double, MODE_A, EPI_RESID>(op, (const c64 *)e, (c64 *)w.r64, (const c64 *)b, nrhs,
                                                 w.part + 2 * pslot, nullptr, w.rn);
@@ -1558,7 +1912,8 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
         if (cyc_prec == MFD_MIXED) target = std::max(target, o.inner_tol);
         long long it = 0;
         int rc;
-        if (cyc_prec == MFD_MIXED)
+        if (method == MFD_BICGSTAB) rc = run_inner_bicgstab(op, nrhs, &o, target, active_d, o.maxit - total_it, &it);
+        else if (cyc_prec == MFD_MIXED)
             rc = run_inner<float>(op, nrhs, &o, target, active_d, o.maxit - total_it, &it, b, e, bn.data());
         else rc = run_inner<double>(op, nrhs, &o, target, active_d, o.maxit - total_it, &it);
         if (rc) return rc;
@@ -1628,6 +1983,7 @@ int solve_batch_dev_impl(int count, mfd_op *const *ops, const void *const *b, vo
         if (!ops[p] || ops[p]->ctx != ctx || !b[p] || !e[p]) { set_error("batched solve: every problem needs an operator of the same context and device b, e"); return MFD_ERR_ARG; }
         for (int q = 0; q < p; ++q)
             if (ops[q] == ops[p]) { set_error("batched solve: an operator appears twice"); return MFD_ERR_ARG; }
+        if (!ops[p]->sym) { set_error("batched solve: SQMR needs complex-symmetric operators (no complex Bloch phase)"); return MFD_ERR_UNSUPPORTED; }
     }
     const int prec0 = o.precision == MFD_MIXED ? MFD_MIXED : MFD_FP64;
     hipStream_t s = ctx->stream;
@@ -1895,6 +2251,10 @@ int solve_impl(mfd_op *op, const mfd_c64 *b, mfd_c64 *e, int64_t nrhs, const mfd
 }
 
 int solve_adjoint_impl(mfd_op *op, const mfd_c64 *gvec, mfd_c64 *lam, const mfd_solve_opts *o, mfd_solve_stats *st) {
+    if (!op->sym) {
+        set_error("adjoint solve: the operator is not complex-symmetric (complex Bloch phase), so Aᵀ is not S A S⁻¹");
+        return MFD_ERR_UNSUPPORTED;
+    }
     int64_t n = 3 * op->G.h.M;
     size_t bytes = sizeof(c64) * n;
     void *db = nullptr, *de = nullptr;
@@ -2068,17 +2428,55 @@ __global__ __launch_bounds__(BS) void k_create_b(DGrid<double> g, const c64 *__r
         b[e] = out;
     }
 }
+
+// HH: b = Cₑ (Pε \ jₑ) − iω jₘ  (model.jl:266-268), one thread per cell
+__global__ __launch_bounds__(BS) void k_create_b_hh(DGrid<double> g, const c64 *__restrict__ eps,
+                                                    const c64 *__restrict__ je, const c64 *__restrict__ jm, c64 iw,
+                                                    int omz, c64 *__restrict__ b) {
+    int64_t p = blockIdx.x * (int64_t)BS + threadIdx.x;
+    if (p >= g.M) return;
+    int i, j, k;
+    cell3(g, p, i, j, k);
+    int id[3] = {i, j, k};
+    auto x = [&](int c, int a) -> c64 {   // (jₑ/ε)_c at p + â (a = -1: at p)
+        int r[3] = {i, j, k};
+        c64 ph = mk<double>(1, 0);
+        if (a >= 0) {
+            r[a] += 1;
+            if (r[a] >= g.N[a]) {
+                if (!g.bloch[a]) return mk<double>(0, 0);
+                r[a] -= g.N[a];
+                ph = g.ph[a];
+            }
+        }
+        int64_t q = r[0] + (int64_t)g.N[0] * (r[1] + (int64_t)g.N[1] * r[2]);
+        c64 v = cdiv(je[dofidx(g, c, q)], eps[q + g.M * (c + 3 * c)]);
+        return v * ph;
+    };
+    for (int u = 0; u < 3; ++u) {
+        int v = u == 0 ? 1 : (u == 1 ? 2 : 0), w = u == 0 ? 2 : (u == 1 ? 0 : 1);
+        // (Cₑ y)_u = ∂_v y_w − ∂_w y_v (forward differences)
+        c64 y = g.sdlinv[DUAL][v][id[v]] * (x(w, v) - x(w, -1)) - g.sdlinv[DUAL][w][id[w]] * (x(v, w) - x(v, -1));
+        int64_t e = dofidx(g, u, p);
+        c64 out = y;
+        if (!omz) out = out - iw * jm[e];
+        b[e] = out;
+    }
+}
 }  // namespace mfd
 
-int create_b_impl(mfd_ctx *ctx, const mfd::DevGrid &G, mfd_c64 omega, const mfd_c64 *mu_arr, const mfd_c64 *je,
+int create_b_impl(mfd_ctx *ctx, const mfd::DevGrid &G, int ft, mfd_c64 omega, const mfd_c64 *par, const mfd_c64 *je,
                   const mfd_c64 *jm, mfd_c64 *b) {
     using namespace mfd;
     int64_t M = G.h.M, n = 3 * M;
     hipStream_t s = ctx->stream;
     DevArray U;
-    int mufull = 0;
-    MFD_TRY(upload_param(ctx, G, mu_arr, U, &mufull));
-    if (mufull) { set_error("create_b: Pμ must be diagonal (Pμ \\ jₘ)"); return MFD_ERR_UNSUPPORTED; }
+    int pfull = 0;
+    MFD_TRY(upload_param(ctx, G, par, U, &pfull));
+    if (pfull) {
+        set_error(ft == EE ? "create_b: Pμ must be diagonal (Pμ \\ jₘ)" : "create_b: Pε must be diagonal (Pε \\ jₑ)");
+        return MFD_ERR_UNSUPPORTED;
+    }
     void *d = nullptr;
     MFD_HIP(hipMalloc(&d, sizeof(c64) * 3 * n));
     c64 *dje = (c64 *)d, *djm = dje + n, *db = djm + n;
@@ -2089,7 +2487,8 @@ int create_b_impl(mfd_ctx *ctx, const mfd::DevGrid &G, mfd_c64 omega, const mfd_
     if (!rc) {
         c64 iw = mk<double>(-omega.im, omega.re);
         int omz = omega.re == 0.0 && omega.im == 0.0;
-        k_create_b<<<(unsigned)((M + BS - 1) / BS), BS, 0, s>>>(G.d64, (const c64 *)U.p, dje, djm, iw, omz, db);
+        if (ft == EE) k_create_b<<<(unsigned)((M + BS - 1) / BS), BS, 0, s>>>(G.d64, (const c64 *)U.p, dje, djm, iw, omz, db);
+        else k_create_b_hh<<<(unsigned)((M + BS - 1) / BS), BS, 0, s>>>(G.d64, (const c64 *)U.p, dje, djm, iw, omz, db);
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(b, db, sizeof(c64) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess)

@@ -16,6 +16,8 @@ struct Scal {
     int conv, done, breakdown, stag;
     long long it, itmax, it_ref;
     double tau_ref;
+    double apend_re, apend_im;   // COCG: α of the step whose iterate update is still pending
+    double om_re, om_im;         // BiCGSTAB: ω
 };
 
 template <typename R>
@@ -48,7 +50,7 @@ struct Workspace {
     int *active = nullptr;    // [3 rhs] per-RHS masks and flags
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0;
-    int graph_kind = 0;       // 1 = SQMR iteration, 2 = COCG iteration
+    int graph_kind = 0;       // 1/3 = SQMR FP32/FP64, 2 = COCG, 4 = BiCGSTAB iteration
     void release();
 };
 
@@ -67,8 +69,10 @@ struct mfd_op {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // tiled stencil shape (TY rows per block, KZ planes per chunk); TY = 0
     // selects the direct kernel
-    int tile64[2] = {4, 16};
-    int tile32[2] = {4, 16};
+    // and XCD-contiguous tile order (1) or plain dispatch order (0)
+    int tile64[3] = {4, 16, 1};
+    int tile32[3] = {4, 16, 1};
+    int sym = 1;              // complex-symmetric up to S_E (no complex Bloch phase)
     int gd = 0;               // grad-div augmented inner operator in use
     int gd_ok = 0;            // eligible (diagonal ε, μ = 1)
 };

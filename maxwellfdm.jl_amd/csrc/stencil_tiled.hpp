@@ -103,21 +103,48 @@ __device__ inline void plane_init(const DGrid<R> &g, int vk, PlaneZ<R> &z) {
     for (int c = 0; c < 3; ++c) z.tinv[c] = g.ssqinv[c == 2 ? DUAL : PRIM][2][z.gk];
 }
 
-template <typename R, int MODE, int EPI, int TY, bool GD>
+// FUSE (COCG step): the operand is p = r + β p_old, formed while staging from
+// r (rall) and p_old (xall); the owner of each point also stores p (pnall) and
+// applies the previous step's deferred iterate update it += α_prev p_old
+// (itall), with β and α_prev taken from the solver scalars.
+template <typename R, int MODE, int EPI, int TY, bool GD, bool FUSE = false>
 __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData<double> od, c64 om2,
                                                          const cx<R> *__restrict__ xall,
                                                          cx<R> *__restrict__ yall, const c64 *__restrict__ ball,
                                                          int64_t vs, int KZ, int nch, double *part,
-                                                         const Scal *scal, const int *__restrict__ mask = nullptr) {
+                                                         const Scal *scal, const int *__restrict__ mask, int xcd,
+                                                         const cx<R> *__restrict__ rall = nullptr,
+                                                         cx<R> *__restrict__ pnall = nullptr,
+                                                         cx<R> *__restrict__ itall = nullptr) {
     using C = TileCfg<TY>;
     constexpr int GB = GD ? 2 : 1, GH = GD ? C::XH : 1, GW = GD ? C::XW : 1;
     __shared__ c64 Xs[2][3][C::XH][C::XW];
     __shared__ c64 Ys[GB][3][GH][GW];
     __shared__ c64 Hs[3][C::HHt][C::HW];
     __shared__ c64 Ps[GB][GD ? C::PHt : 1][GD ? C::PW : 1];
-    const int rhs = blockIdx.z / nch, chunk = blockIdx.z - rhs * nch;
+    // Tile of this block.  Workgroups are dispatched to the 8 XCDs round-robin
+    // (dispatch index L goes to XCD L mod 8); with xcd != 0 each XCD gets one
+    // contiguous run of tiles in x-then-y order, so the y halo rows a tile
+    // stages are read by its neighbour on the same XCD and hit that XCD's L2.
+    const unsigned nbx = gridDim.x, nby = gridDim.y;
+    const unsigned T = nbx * nby * gridDim.z;
+    unsigned L = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z), lt = L;
+    if (xcd) {
+        const unsigned q = T / 8, rem = T % 8, x = L % 8;
+        lt = x * q + (x < rem ? x : rem) + L / 8;
+    }
+    const int bx = (int)(lt % nbx), by = (int)((lt / nbx) % nby), bz = (int)(lt / (nbx * nby));
+    const int rhs = bz / nch, chunk = bz - rhs * nch;
     const cx<R> *x = xall + rhs * vs;
     cx<R> *y = yall + rhs * vs;
+    const cx<R> *rr = FUSE ? rall + rhs * vs : nullptr;
+    cx<R> *pn = FUSE ? pnall + rhs * vs : nullptr;
+    cx<R> *itv = FUSE ? itall + rhs * vs : nullptr;
+    c64 fbeta = czero<double>(), fapend = czero<double>();
+    if (FUSE) {
+        fbeta = mk<double>(scal[rhs].beta_re, scal[rhs].beta_im);
+        fapend = mk<double>(scal[rhs].apend_re, scal[rhs].apend_im);
+    }
     bool skip = false;
     if (EPI == EPI_RESID && mask && !mask[rhs]) return;
     if (EPI == EPI_DOT) {
@@ -125,7 +152,7 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
         skip = scal[rhs].conv != 0;
     }
     const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
-    const int i0 = blockIdx.x * C::TX, j0 = blockIdx.y * TY;
+    const int i0 = bx * C::TX, j0 = by * TY;
     const int i = i0 + tx, j = j0 + ty;
     const bool own = i < g.N[0] && j < g.N[1];
     const int k0 = chunk * KZ, k1 = min(k0 + KZ, g.N[2]);
@@ -215,7 +242,18 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
             for (int c = 0; c < 3; ++c) {
                 c64 v = czero<double>(), yv = czero<double>();
                 if (ok) {
-                    c64 xv = cvt<double>(x[dofidx(g, c, p)]);
+                    const int64_t dof = dofidx(g, c, p);
+                    c64 xv;
+                    if (FUSE) {
+                        c64 po = cvt<double>(x[dof]);
+                        // rounded to the storage precision, so that the operator
+                        // sees exactly the p that is stored
+                        xv = cvt<double>(cvt<R>(cvt<double>(rr[dof]) + fbeta * po));
+                        if (s == 0 && own && vk >= k0 && vk < k1) {   // this block's point
+                            pn[dof] = cvt<R>(xv);
+                            itv[dof] = cvt<R>(cvt<double>(itv[dof]) + fapend * po);
+                        }
+                    } else xv = cvt<double>(x[dof]);
                     c64 qv = cone<double>();
                     if (MODE == MODE_P) {
                         qv = cvt<double>(od.q[3 * p + c]);
@@ -368,6 +406,6 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
         }
     }
     if (EPI == EPI_STORE) return;
-    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * chunk);
+    const int bid = bx + (int)nbx * (by + (int)nby * chunk);
     block_partial<2>(acc, part + (size_t)rhs * NPART * 4, bid);
 }

@@ -14,7 +14,7 @@ LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "lib", "libmfdfd.so"))
 
 MFD_OK, MFD_ERR_ARG, MFD_ERR_UNSUPPORTED, MFD_ERR_HIP, MFD_ERR_NOCONV, MFD_ERR_ALLOC = range(6)
 MFD_FP64, MFD_MIXED = 0, 1
-MFD_SQMR, MFD_COCG = 0, 1
+MFD_SQMR, MFD_COCG, MFD_BICGSTAB = 0, 1, 2
 
 
 class c64(C.Structure):
@@ -85,13 +85,13 @@ SIGNATURES = {
     "mfd_paramop_csc": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int, _P, _P, _P, _P]),
     "mfd_A_nnz": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int, c64, _P, _P, _I64P]),
     "mfd_A_csc": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int, c64, _P, _P, _P, _P, _P]),
-    "mfd_create_b": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int, c64, _P, _P, _P, _P]),
+    "mfd_create_b": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int, c64, _P, _P, _P, _P, _P]),
     "mfd_op_create": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int, c64, _P, _P, C.POINTER(_P)]),
     "mfd_op_destroy": (C.c_int, [_P]),
     "mfd_op_apply": (C.c_int, [_P, _P, _P, C.c_int64]),
     "mfd_op_apply_dev": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int]),
     "mfd_op_set_graddiv": (C.c_int, [_P, C.c_int]),
-    "mfd_op_set_tile": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
+    "mfd_op_set_tile": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "mfd_op_autotune": (C.c_int, [_P, C.c_int64, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                   C.POINTER(C.c_double)]),
     "mfd_op_bench_apply": (C.c_int, [_P, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_double)]),

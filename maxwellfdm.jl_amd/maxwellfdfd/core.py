@@ -392,15 +392,18 @@ def create_A(ft, ω, mdl, εarr=None, μarr=None):
     return to_scipy(A_csc(mdl, ω, εarr, μarr, ft))
 
 
-def create_b(ft, ω, mdl, jₑ, jₘ, μarr=None):
-    """create_b (model.jl:251-274): b = −Cₘ (Pμ \\ jₘ) − iω jₑ on the device."""
+def create_b(ft, ω, mdl, jₑ, jₘ, μarr=None, εarr=None):
+    """create_b (model.jl:251-274) on the device: b = −Cₘ (Pμ \\ jₘ) − iω jₑ
+    (ft = EE) or b = Cₑ (Pε \\ jₑ) − iω jₘ (ft = HH)."""
     μarr = mdl.μarr if μarr is None else μarr
+    εarr = mdl.εarr if εarr is None else εarr
     g = mdl._desc()
     je = L.carr(jₑ)
     jm = L.carr(jₘ)
     b = np.zeros_like(je)
     u = L.carr(μarr, "F")
-    check(lib().mfd_create_b(context(), C.byref(g), ft, L.cplx(ω), ptr(u), ptr(je), ptr(jm), ptr(b)))
+    e = L.carr(εarr, "F")
+    check(lib().mfd_create_b(context(), C.byref(g), ft, L.cplx(ω), ptr(e), ptr(u), ptr(je), ptr(jm), ptr(b)))
     return b
 
 
@@ -491,14 +494,17 @@ class MaxwellOperator:
         """Grad-div augmented inner operator of the solver (see mfd_op_set_graddiv)."""
         check(lib().mfd_op_set_graddiv(self._h, int(bool(enable))))
 
-    def set_tile(self, precision, ty, kz):
-        """Stencil tile shape: TY rows per block (1, 2, 4; 0 = direct kernel), KZ planes per block."""
-        check(lib().mfd_op_set_tile(self._h, L.MFD_FP64 if precision == "fp64" else L.MFD_MIXED, ty, kz))
+    def set_tile(self, precision, ty, kz, xcd_order=1):
+        """Stencil tile shape: TY rows per block (1, 2, 4; 0 = direct kernel), KZ planes per
+        block, and tile order (1: a contiguous run of tiles per XCD, 0: dispatch order)."""
+        check(lib().mfd_op_set_tile(self._h, L.MFD_FP64 if precision == "fp64" else L.MFD_MIXED, ty, kz,
+                                    xcd_order))
 
     def autotune(self, nrhs=1, iters=10):
-        """Pick the stencil tile shape by timing; returns ((TY, KZ) fp64, (TY, KZ) fp32, (ms64, ms32))."""
-        t64 = (C.c_int * 2)()
-        t32 = (C.c_int * 2)()
+        """Pick the stencil tile shape by timing; returns ((TY, KZ, order) fp64,
+        (TY, KZ, order) fp32, (ms64, ms32))."""
+        t64 = (C.c_int * 3)()
+        t32 = (C.c_int * 3)()
         ms = (C.c_double * 2)()
         check(lib().mfd_op_autotune(self._h, nrhs, iters, t64, t32, ms))
         return tuple(t64), tuple(t32), tuple(ms)
@@ -522,7 +528,7 @@ class MaxwellOperator:
         o = L.mfd_solve_opts()
         lib().mfd_solve_opts_default(C.byref(o))
         if method is not None:
-            o.method = {"sqmr": L.MFD_SQMR, "cocg": L.MFD_COCG}[method]
+            o.method = {"sqmr": L.MFD_SQMR, "cocg": L.MFD_COCG, "bicgstab": L.MFD_BICGSTAB}[method]
         o.tol = tol
         o.precision = L.MFD_MIXED if precision == "mixed" else L.MFD_FP64
         if maxit is not None:
@@ -582,6 +588,18 @@ class MaxwellOperator:
         je = None if je is None else L.carr(je)
         check(lib().mfd_e_from_h(self._h, ptr(h), None if je is None else ptr(je), ptr(e)))
         return e
+
+    def solve_with_pullback(self, b, tol=1e-10, precision="fp64"):
+        """The rrule of ε ↦ e = A(ε)⁻¹ b: returns e and pullback(g), which for
+        g = (∂f/∂e)ᵀ of a real objective f solves Aᵀλ = g and returns ∂f/∂ε per
+        E DOF (2 Re(ω² λ ∘ e))."""
+        e, _ = self.solve(b, tol=tol, precision=precision)
+
+        def pullback(g, tol_adjoint=tol):
+            lam, _ = self.solve_adjoint(g, tol=tol_adjoint)
+            return self.adjoint_grad(e, lam)
+
+        return e, pullback
 
     def adjoint_grad(self, e, lam):
         """∂f/∂ε per E DOF: 2 Re(ω² λ ∘ e) for Aᵀλ = (∂f/∂e)ᵀ."""

@@ -86,3 +86,32 @@ def uneven_problems(count=32, nmin=48, nmax=128, npml=10, nbox=40, seed=7):
         m, _ = random_boxes(N=N, npml=npml, nbox=nbox, lam=2 * np.pi / om, seed=seed * 1000 + p, ndip=1)
         out.append((m, om))
     return out
+
+
+def adjoint_design(N=(128, 128, 64), npml=10, design=(64, 64, 16), lam=24.0, seed=3, monitor=(4, 4, 1)):
+    """Adjoint case: vacuum with a centred design region whose ε per voxel is
+    1 + 11·sigmoid(G∗ξ), ξ ~ N(0,1) seeded and G a Gaussian of σ = 3 cells
+    (applied to all three diagonal components); a z-dipole below the region and
+    a monitor patch of E_z DOFs above it.  Returns (model with εarr assigned,
+    ω, b, monitor DOF indices, design-region cell ranges)."""
+    from scipy.ndimage import gaussian_filter
+    om = 2 * np.pi / lam
+    m = base_model(N, npml, om)
+    M.calc_matparams(m)
+    rng = np.random.default_rng(seed)
+    lo = [(n - d) // 2 for n, d in zip(N, design)]
+    xi = gaussian_filter(rng.normal(size=design), sigma=3.0, mode="nearest")
+    xi /= xi.std()
+    eps = 1.0 + 11.0 / (1.0 + np.exp(-xi))
+    sl = tuple(slice(l, l + d) for l, d in zip(lo, design))
+    for c in range(3):
+        m.εarr[sl + (c, c)] = eps
+    zsrc = max(npml + 2, lo[2] - 4)
+    M.add_srce(m, M.PointSrc((N[0] / 2, N[1] / 2, zsrc), (0, 0, 1), 1.0))
+    je, jm = M.create_srcs(m)
+    b = M.create_b(M.EE, om, m, je, jm, μarr=m.μarr)
+    zmon = min(N[2] - npml - 2, lo[2] + design[2] + 4)
+    i0, j0 = N[0] // 2 - monitor[0] // 2, N[1] // 2 - monitor[1] // 2
+    mon = [2 + 3 * (i + N[0] * (j + N[1] * k)) for k in range(zmon, zmon + monitor[2])
+           for j in range(j0, j0 + monitor[1]) for i in range(i0, i0 + monitor[0])]
+    return m, om, b, np.array(mon), sl

@@ -18,6 +18,9 @@
  *   distweights        source.jl:208-306, line by line.  Pinned by the golden
  *                      vectors of test/source.jl:17-133.
  *   point source       pointsrc.jl:64-103.  Pinned by test/source.jl:184-242.
+ *   plane source       planesrc.jl:41-79.  Pinned by test/source.jl:123-170
+ *                      (peak J = K/∆z on the 1 and 0.5 grids, zero Jy/Jz,
+ *                      resolution-independent total current).
  *   curl               model.jl:160-175 -> MaxwellBase create_curl (absent);
  *                      Yee forward/backward differences.  Pinned by div∘curl=0,
  *                      Cm = S_E^-1 Ce^T S_H, the periodic plane-wave eigenvalue
@@ -517,8 +520,9 @@ static void sub_dropzeros(int64_t n, const og_csc *A, oc64 s, const og_csc *B, o
     C->nnz = nz; C->colptr = colptr; C->rowval = rowval; C->nzval = nzval;
 }
 
+/* diagonal of a diagonal parameter operator in DOF order (Pμ for EE, Pε for HH) */
 static int mu_diag(const G *g, const oc64 *mu_arr, oc64 *md) {
-    if (has_offdiag(g, mu_arr)) return -3;   /* Pμ \ Cₑ needs a diagonal Pμ (model.jl:236) */
+    if (has_offdiag(g, mu_arr)) return -3;   /* P \ C needs a diagonal P (model.jl:236, 239) */
     for (int c = 0; c < 3; ++c)
         for (int64_t p = 0; p < g->M; ++p) md[dof(g, c, p)] = mu_arr[p + g->M * (c + 3 * c)];
     return 0;
@@ -550,6 +554,32 @@ int og_create_A(const og_desc *d, double om_re, double om_im, const oc64 *eps_ar
     return 0;
 }
 
+/* model.jl:238-240: A = Cₑ*(Pε\Cₘ); iszero(ω) || (A -= ω^2*Pμ) (needs a diagonal Pε) */
+int og_create_A_hh(const og_desc *d, double om_re, double om_im, const oc64 *eps_arr,
+                   const oc64 *mu_arr, og_csc *out) {
+    G g; int rc = g_build(d, &g); if (rc) return rc;
+    int64_t n = 3 * g.M;
+    oc64 *ed = malloc(sizeof(oc64) * n);
+    rc = mu_diag(&g, eps_arr, ed);
+    if (rc) { free(ed); g_free(&g); return rc; }
+    tlist L = {0};
+    og_csc Ce, Cm, C;
+    curl_trip(&g, 0, &L); csc_from_trip(n, &L, &Ce); L.n = 0;
+    curl_trip(&g, 1, &L); csc_from_trip(n, &L, &Cm); L.n = 0;
+    for (int64_t e = 0; e < Cm.nnz; ++e) Cm.nzval[e] = Cm.nzval[e] / ed[Cm.rowval[e] - 1];
+    spgemm(n, &Ce, &Cm, &C);
+    oc64 om = om_re + om_im * I;
+    if (om != 0) {
+        og_csc Pm;
+        paramop_trip(&g, 1, mu_arr, &L); csc_from_trip(n, &L, &Pm);
+        sub_dropzeros(n, &C, om * om, &Pm, out);
+        og_csc_free(&Pm); og_csc_free(&C);
+    } else *out = C;
+    og_csc_free(&Ce); og_csc_free(&Cm);
+    free(L.t); free(ed); g_free(&g);
+    return 0;
+}
+
 int og_spmv(int64_t n, const og_csc *A, const oc64 *x, oc64 *y) {
     for (int64_t i = 0; i < n; ++i) y[i] = 0;
     for (int64_t j = 0; j < n; ++j)
@@ -576,6 +606,25 @@ int og_create_b(const og_desc *d, double om_re, double om_im, const oc64 *mu_arr
         if (om_re != 0 || om_im != 0) b[i] = b[i] - iw * je[i];
     }
     og_csc_free(&Cm); free(L.t); free(md); free(y); g_free(&g);
+    return 0;
+}
+
+/* model.jl:266-268: b = Cₑ*(Pε\jₑ); iszero(ω) || (b .-= (im*ω).*jₘ) */
+int og_create_b_hh(const og_desc *d, double om_re, double om_im, const oc64 *eps_arr,
+                   const oc64 *je, const oc64 *jm, oc64 *b) {
+    G g; int rc = g_build(d, &g); if (rc) return rc;
+    int64_t n = 3 * g.M;
+    oc64 *ed = malloc(sizeof(oc64) * n), *y = malloc(sizeof(oc64) * n);
+    rc = mu_diag(&g, eps_arr, ed);
+    if (rc) { free(ed); free(y); g_free(&g); return rc; }
+    tlist L = {0}; og_csc Ce;
+    curl_trip(&g, 0, &L); csc_from_trip(n, &L, &Ce);
+    for (int64_t i = 0; i < n; ++i) y[i] = je[i] / ed[i];
+    og_spmv(n, &Ce, y, b);
+    oc64 iw = (om_re + om_im * I) * I;
+    if (om_re != 0 || om_im != 0)
+        for (int64_t i = 0; i < n; ++i) b[i] = b[i] - iw * jm[i];
+    og_csc_free(&Ce); free(L.t); free(ed); free(y); g_free(&g);
     return 0;
 }
 

@@ -75,6 +75,11 @@ int og_create_A(const og_desc *d, double om_re, double om_im, const oc64 *eps_ar
                 const oc64 *mu_arr, og_csc *out);
 int og_create_b(const og_desc *d, double om_re, double om_im, const oc64 *mu_arr,
                 const oc64 *je, const oc64 *jm, oc64 *b);
+/* HH formulation (model.jl:238-240, 266-268); Pε must be diagonal */
+int og_create_A_hh(const og_desc *d, double om_re, double om_im, const oc64 *eps_arr,
+                   const oc64 *mu_arr, og_csc *out);
+int og_create_b_hh(const og_desc *d, double om_re, double om_im, const oc64 *eps_arr,
+                   const oc64 *je, const oc64 *jm, oc64 *b);
 void og_csc_free(og_csc *m);
 int og_spmv(int64_t n, const og_csc *A, const oc64 *x, oc64 *y);
 

@@ -177,27 +177,30 @@ def paramop(spec, which, parr):
     return _take_csc(m, n)
 
 
-def create_A(spec, omega, eps_arr, mu_arr):
+def create_A(spec, omega, eps_arr, mu_arr, ft=0):
+    """model.jl:225-246; ft = 0 (EE) or 1 (HH)."""
     d = _desc(spec)
     e = np.asfortranarray(eps_arr, dtype=np.complex128)
     u = np.asfortranarray(mu_arr, dtype=np.complex128)
     n = 3 * int(np.prod(_N(spec)))
     m = CSC()
     om = complex(omega)
-    _check(lib().og_create_A(C.byref(d), C.c_double(om.real), C.c_double(om.imag), _ptr(e), _ptr(u),
-                             C.byref(m)), "create_A")
+    f = lib().og_create_A_hh if ft else lib().og_create_A
+    _check(f(C.byref(d), C.c_double(om.real), C.c_double(om.imag), _ptr(e), _ptr(u), C.byref(m)), "create_A")
     return _take_csc(m, n)
 
 
-def create_b(spec, omega, mu_arr, je, jm):
+def create_b(spec, omega, mu_arr, je, jm, ft=0, eps_arr=None):
+    """model.jl:251-274; EE uses mu_arr, HH uses eps_arr."""
     d = _desc(spec)
-    u = np.asfortranarray(mu_arr, dtype=np.complex128)
+    p = np.asfortranarray(eps_arr if ft else mu_arr, dtype=np.complex128)
     je = np.ascontiguousarray(je, dtype=np.complex128)
     jm = np.ascontiguousarray(jm, dtype=np.complex128)
     b = np.zeros_like(je)
     om = complex(omega)
-    _check(lib().og_create_b(C.byref(d), C.c_double(om.real), C.c_double(om.imag), _ptr(u), _ptr(je),
-                             _ptr(jm), _ptr(b)), "create_b")
+    f = lib().og_create_b_hh if ft else lib().og_create_b
+    _check(f(C.byref(d), C.c_double(om.real), C.c_double(om.imag), _ptr(p), _ptr(je), _ptr(jm), _ptr(b)),
+           "create_b")
     return b
 
 

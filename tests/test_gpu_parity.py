@@ -126,15 +126,33 @@ def test_A_csc(case, full, pkg, O):
         assert_csc_equal(pkg.A_csc(m, om, eps, mu), O.create_A(s, om, eps, mu), 1e-13, "A")
 
 
-def test_A_rejects_nondiagonal_mu_and_HH(pkg):
+@pytest.mark.parametrize("case", list(CASES))
+def test_A_csc_hh(case, pkg, O):
+    """HH formulation: A = Cₑ (Pε \\ Cₘ) − ω² Pμ (full μ allowed), bit-exact structure."""
+    m = make(pkg, case)
+    s = spec_of(m)
+    mu, eps = rand_params(m, 14, full=True, mu=True)   # ε diagonal, μ with off-diagonals
+    for om in (0.55, 0.0):
+        say("A_csc HH", case, om)
+        assert_csc_equal(pkg.A_csc(m, om, eps, mu, ft=pkg.HH), O.create_A(s, om, eps, mu, ft=1), 1e-13, "A_HH")
+
+
+def test_A_rejects_nondiagonal_divisor(pkg):
     m = make(pkg, "uniform_pml")
     eps, mu = rand_params(m, 5)
     mu2 = mu.copy()
     mu2[..., 0, 1] = 0.1
     with pytest.raises(pkg.MfdUnsupportedError):
         pkg.A_csc(m, 0.5, eps, mu2)
+    eps2 = eps.copy()
+    eps2[..., 0, 2] = 0.1
     with pytest.raises(pkg.MfdUnsupportedError):
-        pkg.A_csc(m, 0.5, eps, mu, ft=pkg.HH)
+        pkg.A_csc(m, 0.5, eps2, mu, ft=pkg.HH)
+    with pytest.raises(pkg.MfdUnsupportedError):
+        pkg.create_b(pkg.HH, 0.5, m, np.zeros(3 * m.grid.M, complex), np.zeros(3 * m.grid.M, complex), εarr=eps2)
+    mb = make(pkg, "boundft_HH")
+    with pytest.raises(pkg.MfdUnsupportedError):
+        pkg.create_b(pkg.EE, 0.5, mb, np.zeros(3 * mb.grid.M, complex), np.zeros(3 * mb.grid.M, complex))
     m2 = make(pkg, "uniform_pml")
     pkg.set_ωpml(m2, 0.0)
     with pytest.raises(pkg.MfdArgumentError):
@@ -154,6 +172,9 @@ def test_create_b(case, pkg, O):
         say("create_b", case, om)
         got = pkg.create_b(pkg.EE, om, m, je, jm, μarr=mu)
         ref = O.create_b(s, om, mu, je, jm)
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+        got = pkg.create_b(pkg.HH, om, m, je, jm, εarr=eps)
+        ref = O.create_b(s, om, mu, je, jm, ft=1, eps_arr=eps)
         assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
@@ -191,16 +212,18 @@ OPCASES = ["uniform_pml", "nonuniform_bloch", "mixed_pml_bloch", "tiny_bloch_axe
 
 
 @pytest.mark.parametrize("case", ["wide", "wide_bloch", "nonuniform_bloch", "tiny_bloch_axes"])
-@pytest.mark.parametrize("ty,kz", [(1, 1), (1, 4), (2, 3), (2, 16), (4, 2), (4, 16), (0, 1)])
-def test_apply_tile_shapes(case, ty, kz, pkg, O):
-    """Every tile shape of the stencil (and the direct kernel, TY = 0) gives the CSC product."""
+@pytest.mark.parametrize("ty,kz,xo", [(1, 1, 1), (1, 4, 0), (2, 3, 1), (2, 16, 0), (4, 2, 1), (4, 16, 0),
+                                      (4, 16, 1), (8, 3, 1), (8, 16, 0), (0, 1, 0)])
+def test_apply_tile_shapes(case, ty, kz, xo, pkg, O):
+    """Every tile shape and tile order of the stencil (and the direct kernel,
+    TY = 0) gives the CSC product."""
     m = make(pkg, case)
     eps, muu = rand_params(m, 8)
     om = 0.55
-    say("tiles", case, ty, kz)
+    say("tiles", case, ty, kz, xo)
     A = O.to_scipy(O.create_A(spec_of(m), om, eps, muu))
     op = pkg.MaxwellOperator(m, om, εarr=eps, μarr=muu)
-    op.set_tile("fp64", ty, kz)
+    op.set_tile("fp64", ty, kz, xo)
     rng = np.random.default_rng(4)
     n = 3 * m.grid.M
     X = rng.normal(size=(2, n)) + 1j * rng.normal(size=(2, n))
@@ -292,6 +315,87 @@ def test_solve_vs_direct(precision, pkg, O):
     assert st.converged and rel <= 1.05e-8
     xd = sla.spsolve(A.tocsc(), b)
     assert np.linalg.norm(e - xd) / np.linalg.norm(xd) <= 1e-6
+    op.close()
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(use_graph=False, check_every=5), dict(check_every=7),
+                                dict(precision="mixed")], ids=["graph", "nograph_odd", "graph_odd", "mixed"])
+def test_solve_cocg(kw, pkg, O):
+    """COCG (fused step: p formed in the stencil, deferred iterate update) against
+    the direct solve, single and batched RHS."""
+    m, om, b = c1_like(pkg)
+    A = O.to_scipy(O.create_A(spec_of(m), om, m.εarr, m.μarr))
+    op = pkg.MaxwellOperator(m, om)
+    say("cocg", kw)
+    e, st = op.solve(b, tol=1e-8, method="cocg", **kw)
+    rel = np.linalg.norm(b - A @ e) / np.linalg.norm(b)
+    say("  iterations", st.iterations, "updates", st.outer_cycles, "oracle residual", rel)
+    assert st.converged and rel <= 1.05e-8
+    xd = sla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(e - xd) / np.linalg.norm(xd) <= 1e-6
+    B = np.array([b, 1j * b[::-1].copy()])
+    E, st = op.solve(B, tol=1e-8, method="cocg", **kw)
+    for r in range(2):
+        assert np.linalg.norm(B[r] - A @ E[r]) / np.linalg.norm(B[r]) <= 1.05e-8
+    op.close()
+
+
+@pytest.mark.parametrize("case", ["uniform_pml", "cmplast"])
+def test_solve_cocg_direct_kernel(case, pkg, O):
+    """COCG on operators served by the direct stencil (off-diagonal ε, μ ≠ 1),
+    where p and the deferred update run as their own kernel."""
+    m = make(pkg, case)
+    eps, muu = rand_params(m, 5, full=True, mu=True)
+    om = 0.45
+    A = O.to_scipy(O.create_A(spec_of(m), om, eps, muu))
+    op = pkg.MaxwellOperator(m, om, εarr=eps, μarr=muu)
+    rng = np.random.default_rng(6)
+    n = 3 * m.grid.M
+    b = rng.normal(size=n) + 1j * rng.normal(size=n)
+    for meth in ("cocg", "sqmr"):
+        say("direct kernel", case, meth)
+        e, st = op.solve(b, tol=1e-8, method=meth)
+        assert np.linalg.norm(b - A @ e) / np.linalg.norm(b) <= 1.05e-8
+    op.close()
+
+
+@pytest.mark.parametrize("case", ["nonuniform_bloch", "mixed_pml_bloch"])
+@pytest.mark.parametrize("full", [False, True])
+def test_solve_bloch_bicgstab(case, full, pkg, O):
+    """A complex Bloch phase makes A non-symmetric: the default solve uses
+    BiCGSTAB; SQMR/COCG and the transpose-based adjoint refuse."""
+    m = make(pkg, case)
+    eps, muu = rand_params(m, 12, full=full, mu=full)
+    om = 0.55 + 0.02j
+    A = O.to_scipy(O.create_A(spec_of(m), om, eps, muu))
+    op = pkg.MaxwellOperator(m, om, εarr=eps, μarr=muu)
+    rng = np.random.default_rng(13)
+    n = 3 * m.grid.M
+    B = rng.normal(size=(2, n)) + 1j * rng.normal(size=(2, n))
+    say("bloch bicgstab", case, full)
+    for prec in ("fp64", "mixed"):
+        E, st = op.solve(B, tol=1e-8, precision=prec)
+        for r in range(2):
+            rel = np.linalg.norm(B[r] - A @ E[r]) / np.linalg.norm(B[r])
+            assert st.converged and rel <= 1.05e-8
+        assert st.iterations_fp32 == 0
+    xd = sla.spsolve(A.tocsc(), B[0])
+    assert np.linalg.norm(E[0] - xd) / np.linalg.norm(xd) <= 1e-6
+    for meth in ("sqmr", "cocg"):
+        with pytest.raises(pkg.MfdUnsupportedError):
+            op.solve(B[0], method=meth)
+    with pytest.raises(pkg.MfdUnsupportedError):
+        op.solve_adjoint(B[0])
+    op.close()
+
+
+def test_solve_bicgstab_symmetric(pkg, O):
+    """BiCGSTAB chosen explicitly on a symmetric operator, against the direct solve."""
+    m, om, b = c1_like(pkg)
+    A = O.to_scipy(O.create_A(spec_of(m), om, m.εarr, m.μarr))
+    op = pkg.MaxwellOperator(m, om)
+    e, st = op.solve(b, tol=1e-8, method="bicgstab", use_graph=False, check_every=7)
+    assert st.converged and np.linalg.norm(b - A @ e) / np.linalg.norm(b) <= 1.05e-8
     op.close()
 
 
@@ -446,4 +550,46 @@ def test_adjoint(pkg, O):
         fd = (f_of(ep) - f_of(em)) / (2 * h)
         dof = c + 3 * (i + N[0] * (j + N[1] * k))
         say("  fd", fd, "adjoint", grad[dof])
+        assert grad[dof] == pytest.approx(fd, rel=1e-5, abs=1e-9 * np.abs(grad).max())
+
+
+def test_adjoint_design_region(pkg, O):
+    """Adjoint case (design region ε = 1 + 11·sigmoid(G∗ξ), dipole, E_z monitor)
+    at a reduced size: the pullback against the CPU FP64 adjoint (direct solve of
+    the oracle's Aᵀ) and against central finite differences (h = 1e-4)."""
+    from maxwellfdfd import workloads as W
+    m, om, b, mon, sl = W.adjoint_design(N=(32, 32, 16), npml=4, design=(16, 16, 4), lam=16.0, seed=3,
+                                         monitor=(2, 2, 1))
+    A = O.to_scipy(O.create_A(spec_of(m), om, m.εarr, m.μarr)).tocsc()
+    op = pkg.MaxwellOperator(m, om)
+    say("adjoint design: forward")
+    e, pullback = op.solve_with_pullback(b, tol=1e-12)
+    g = np.zeros_like(e)
+    g[mon] = np.conj(e[mon])
+    say("adjoint design: pullback")
+    grad = pullback(g)
+    lam_ref = sla.spsolve(A.T.tocsc(), g)
+    x_ref = sla.spsolve(A, b)
+    grad_ref = 2 * np.real(om ** 2 * lam_ref * x_ref)
+    assert np.linalg.norm(grad - grad_ref) <= 1e-8 * np.linalg.norm(grad_ref)
+    op.close()
+    N = m.grid.N
+
+    def f_of(eps_arr):
+        o = pkg.MaxwellOperator(m, om, εarr=eps_arr)
+        x, _ = o.solve(b, tol=1e-13, x0=e)
+        o.close()
+        return np.sum(np.abs(x[mon]) ** 2)
+
+    rng = np.random.default_rng(1)
+    h = 1e-4
+    for t in range(4):
+        c = int(rng.integers(3))
+        i, j, k = (int(rng.integers(s.start, s.stop)) for s in sl)
+        ep, em = m.εarr.copy(), m.εarr.copy()
+        ep[i, j, k, c, c] += h
+        em[i, j, k, c, c] -= h
+        fd = (f_of(ep) - f_of(em)) / (2 * h)
+        dof = c + 3 * (i + N[0] * (j + N[1] * k))
+        say("  voxel", (i, j, k, c), "fd", fd, "adjoint", grad[dof])
         assert grad[dof] == pytest.approx(fd, rel=1e-5, abs=1e-9 * np.abs(grad).max())

@@ -256,6 +256,41 @@ def test_A_is_product_of_factors(full, O):
     np.testing.assert_allclose(b, -(Cm @ (jm / Pm.diagonal())) - 1j * om * je, rtol=1e-12, atol=1e-12)
 
 
+def test_A_hh_is_product_of_factors(O):
+    """HH formulation (model.jl:238-240, 266-268): A = Cₑ Pε⁻¹ Cₘ − ω² Pμ with
+    Aᵀ = S_H A S_H⁻¹, and b = Cₑ Pε⁻¹ jₑ − iω jₘ."""
+    N = (4, 5, 4)
+    s = spec(N, npml=1, om=0.5, lprim=nonuniform(N, 13))
+    rng = np.random.default_rng(8)
+    eps = np.zeros(N + (3, 3), dtype=complex, order="F")
+    mu = np.zeros_like(eps)
+    for c in range(3):
+        eps[..., c, c] = 1 + 11 * rng.random(N) + 0.2j * rng.random(N)
+        mu[..., c, c] = 1 + 0.5 * rng.random(N)
+    om = 0.45 + 0.01j
+    A = O.to_scipy(O.create_A(s, om, eps, mu, ft=1))
+    Ce = O.to_scipy(O.curl(s, 0))
+    Cm = O.to_scipy(O.curl(s, 1))
+    Pe = O.to_scipy(O.paramop(s, 0, eps))
+    Pm = O.to_scipy(O.paramop(s, 1, mu))
+    ref = Ce @ sp.diags(1 / Pe.diagonal()) @ Cm - om ** 2 * Pm
+    assert abs(A - ref).max() <= 1e-13 * abs(A).max()
+    SH = S_of(tables(O, s), N, 1)
+    assert abs(A.T - sp.diags(SH) @ A @ sp.diags(1 / SH)).max() <= 1e-12 * abs(A).max()
+    je = rng.normal(size=A.shape[0]) + 1j * rng.normal(size=A.shape[0])
+    jm = rng.normal(size=A.shape[0]) + 1j * rng.normal(size=A.shape[0])
+    b = O.create_b(s, om, mu, je, jm, ft=1, eps_arr=eps)
+    np.testing.assert_allclose(b, Ce @ (je / Pe.diagonal()) - 1j * om * jm, rtol=1e-12, atol=1e-12)
+    # the E and H solutions of one source pair are linked by h = (i/ω) μ⁻¹ (Cₑ e + jₘ)
+    bE = O.create_b(s, om, mu, je, jm)
+    AE = O.to_scipy(O.create_A(s, om, eps, mu))
+    import scipy.sparse.linalg as spla
+    e = spla.spsolve(AE.tocsc(), bE)
+    h = spla.spsolve(A.tocsc(), b)
+    h_from_e = (1j / om) * ((Ce @ e + jm) / Pm.diagonal())
+    assert np.linalg.norm(h - h_from_e) <= 1e-9 * np.linalg.norm(h)
+
+
 def test_A_omega_zero_keeps_structure(O):
     s = spec(4, npml=0)
     eps = np.zeros((4, 4, 4, 3, 3), dtype=complex, order="F")
