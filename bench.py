@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_c2.json"))
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the per-kernel timing launches and a 256 MiB device copy (for hardware-counter runs)")
+    ap.add_argument("--tiles", default=None,
+                    help="fixed stencil tiles 'TY,KZ,order/TY,KZ,order' (FP64/FP32) instead of the autotuner, "
+                         "so that separate hardware-counter passes launch the same kernels")
     ap.add_argument("--workload", default="random-boxes",
                     choices=["random-boxes", "batched-rhs", "uneven-batch", "adjoint"],
                     help="random-boxes: the metric line; the others time the batched-RHS, uneven-problem "
@@ -132,10 +135,12 @@ def run_batched_rhs(a):
     b_d = torch.from_numpy(B).to(dev)
     e_d = torch.zeros_like(b_d)
     torch.cuda.synchronize()
+    print(f"[batched-rhs] {N}^3, {a.nrhs} right-hand sides: batched solve", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     st = op.solve_dev(b_d.data_ptr(), e_d.data_ptr(), a.nrhs, tol=1e-8)
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
+    print(f"[batched-rhs] batched solve {t_batch:.2f} s; single-RHS solve", file=sys.stderr, flush=True)
     e1 = torch.zeros_like(b_d[0])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -159,6 +164,7 @@ def run_uneven_batch(a):
     import maxwellfdfd as M
     from maxwellfdfd import workloads as W
     probs = W.uneven_problems(count=a.count, nmin=48, nmax=128, npml=10, nbox=40, seed=7)
+    log = lambda *x: print("[uneven-batch]", *x, file=sys.stderr, flush=True)
     dev = torch.device("cuda:0")
     ops, bds, eds, cells = [], [], [], 0
     for m, om in probs:
@@ -171,16 +177,22 @@ def run_uneven_batch(a):
         bds.append(torch.from_numpy(b).to(dev))
         eds.append(torch.zeros_like(bds[-1]))
         cells += m.grid.M
+        log("set up problem", len(ops), "N =", m.grid.N[0])
     torch.cuda.synchronize()
+    log("batched solve of", len(ops), "problems,", cells, "cells")
+    os.environ["MFD_SOLVE_TRACE"] = "1"   # the solver reports each problem as it converges
     t0 = time.perf_counter()
     st = ops[0].solve_batch_dev(ops[1:], [x.data_ptr() for x in bds], [x.data_ptr() for x in eds], tol=1e-8)
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
+    del os.environ["MFD_SOLVE_TRACE"]
+    log("batched solve done in", round(t_batch, 2), "s; sequential solves")
     t0 = time.perf_counter()
     its = []
     for op, bd, ed in zip(ops, bds, eds):
         ed.zero_()
         its.append(int(op.solve_dev(bd.data_ptr(), ed.data_ptr(), 1, tol=1e-8).iterations))
+        log("sequential problem", len(its), "iterations", its[-1])
     torch.cuda.synchronize()
     t_seq = time.perf_counter() - t0
     for op in ops:
@@ -216,6 +228,8 @@ def run_adjoint(a):
     rng = np.random.default_rng(1)
     h = 1e-4
     errs = []
+    print(f"[adjoint] forward {t_fwd:.2f} s, adjoint + gradient {t_adj:.2f} s; finite differences",
+          file=sys.stderr, flush=True)
     for _ in range(a.voxels):
         c = int(rng.integers(3))
         i, j, k = (int(rng.integers(s.start, s.stop)) for s in sl)
@@ -265,7 +279,13 @@ def main():
     je, jm = M.create_srcs(m)
     b = M.create_b(M.EE, om, m, je, jm)
     op = M.MaxwellOperator(m, om)
-    tiles = op.autotune(1, 10)
+    if a.tiles:
+        t64, t32 = ([int(v) for v in t.split(",")] for t in a.tiles.split("/"))
+        op.set_tile("fp64", *t64)
+        op.set_tile("mixed", *t32)
+        tiles = (tuple(t64), tuple(t32), None)
+    else:
+        tiles = op.autotune(1, 10)
     cells = m.grid.M
     dev = torch.device("cuda:0")
     if a.kernels_only:

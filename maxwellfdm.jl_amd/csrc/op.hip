@@ -598,7 +598,7 @@ __global__ void k_jacobi(DGrid<double> g, OpData<double> od, c64 *q, int use) {
 
 // grad-div weight W = 1/ε̄² at each primal node, ε̄ the mean of the ε_ww of
 // the (up to 6) E edges meeting at the node
-__global__ void k_wnode(DGrid<double> g, const c64 *epsd, c64 *w, c32 *w32, int wmode) {
+__global__ void k_wnode(DGrid<double> g, const c64 *epsd, c64 *w, c32 *w32) {
     int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (p >= g.M) return;
     int id[3];
@@ -613,7 +613,7 @@ __global__ void k_wnode(DGrid<double> g, const c64 *epsd, c64 *w, c32 *w32, int 
         else if (g.bloch[a]) { s = s + epsd[3 * (p + (int64_t)(g.N[a] - 1) * st) + a]; ++n; }
     }
     c64 m = mk<double>(s.re / n, s.im / n);
-    c64 v = wmode == 1 ? cinv(m) : (wmode == 2 ? mk<double>(1, 0) : cinv(m * m));
+    c64 v = cinv(m * m);   // W = 1/ε̄² at the node
     w[p] = v;
     w32[p] = mk<float>((float)v.re, (float)v.im);
 }
@@ -909,8 +909,7 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
         if (op->G.d64.bloch[a] && fabs(op->G.d64.ph[a].im) > 1e-14) op->sym = 0;
     op->gd_ok = !full && !op->has_mu && !(om2.re == 0.0 && om2.im == 0.0);
     op->gd = 0;   // opt-in: it helps small PML-dominated grids and slows larger ones
-    const char *wm = getenv("MFD_GD_WMODE");
-    k_wnode<<<nb(M), BS, 0, s>>>(op->G.d64, epsd, wn, wn32, wm ? atoi(wm) : 0);
+    k_wnode<<<nb(M), BS, 0, s>>>(op->G.d64, epsd, wn, wn32);
     if (op->gd) k_jacobi_gd<<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q);
     else if (op->has_mu) k_jacobi<true><<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q, 1);
     else k_jacobi<false><<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q, 1);
@@ -1529,14 +1528,12 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
         w.graph_kind = kind;
     }
     long long done_it = 0;
-    const char *pe = getenv("MFD_PROBE_EVERY");
-    const long long probe = std::max<long long>(chunk, pe ? atoll(pe) : 512) / chunk * chunk;
+    // FP32 cycles probe the true FP64 residual every `probe` steps and end after
+    // two probes in a row that fail to halve it
+    const long long probe = std::max<long long>(chunk, 512) / chunk * chunk;
+    const int stall_max = 2;
     double best_probe = 1e300;
     int stall = 0;
-    const char *pst = getenv("MFD_PROBE_STALL");
-    const int stall_max = pst ? atoi(pst) : 2;
-    const char *te = getenv("MFD_TRACE_EVERY");
-    const long long trace_every = te ? atoll(te) / chunk * chunk : 0;
     while (true) {
         if (use_graph) MFD_HIP(hipGraphLaunch(w.graph, s));
         else
@@ -1553,8 +1550,6 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
         }
         *iters_out = mx;
         if (all || mx >= budget) break;
-        if (trace_every > 0 && done_it % trace_every == 0)
-            fprintf(stderr, "[mfd iter] it %lld tau/cn %.3e\n", mx, w.scal_h[0].tau / w.scal_h[0].cn);
         // FP32 cycles: probe the true FP64 residual every `probe` iterations and
         // end the cycle once it stops improving (2x per probe window)
         if (sizeof(R) == 4 && b && e && done_it % probe == 0) {
@@ -2047,7 +2042,13 @@ int solve_batch_dev_impl(int count, mfd_op *const *ops, const void *const *b, vo
             MFD_HIP(hipStreamSynchronize(s));
             double rel = B.bn > 0 ? sqrt(w.rn_h[0] / B.bn) : 0.0;
             B.worst = rel;
-            if (rel <= o.tol) { B.conv = B.done = true; return MFD_OK; }
+            if (rel <= o.tol) {
+                B.conv = B.done = true;
+                if (getenv("MFD_SOLVE_TRACE"))
+                    fprintf(stderr, "[mfd batch] a problem of %lld cells converged after %lld iterations\n",
+                            (long long)op->G.h.M, B.it_total);
+                return MFD_OK;
+            }
             if (B.it_total >= o.maxit || B.cycles > 200) { B.done = true; return MFD_OK; }
             if (B.prec == MFD_MIXED && B.cycles > 0 && rel > 0.1 * B.prev) B.prec = MFD_FP64;
             B.prev = rel;
