@@ -81,7 +81,12 @@ def cpu_baseline(m, om, fp64_iters, nthreads, iters):
     et, mt = [], []
     for o in m.objs:
         s = o.shape
-        shapes.append(("ball", tuple(s.c), s.r) if hasattr(s, "r") else ("cuboid", tuple(s.c), tuple(s.s)))
+        if hasattr(s, "h"):
+            shapes.append(("cylinder", tuple(s.c), s.r, s.h))
+        elif hasattr(s, "r"):
+            shapes.append(("ball", tuple(s.c), s.r))
+        else:
+            shapes.append(("cuboid", tuple(s.c), tuple(s.s)))
         et.append(o.mat.eps)
         mt.append(o.mat.mu)
     t0 = time.perf_counter()

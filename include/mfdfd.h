@@ -58,7 +58,8 @@ typedef struct {
     int32_t order_cmpfirst;   /* model.jl:72 */
 } mfd_grid;
 
-/* Shapes of add_obj! (model.jl:107-120): kind 0 = ball (c, r), 1 = cuboid (c, s). */
+/* Shapes of add_obj! (model.jl:107-120): kind 0 = ball (c, r), 1 = cuboid (c, s),
+ * 2 = cylinder along z (c, r, height s[2]; a 2D disk extruded). */
 typedef struct {
     int32_t kind;
     int32_t pad;

@@ -30,6 +30,10 @@ __device__ inline bool inside(const DShape &s, const double x[3]) {
         double a = x[0] - s.c[0], b = x[1] - s.c[1], c = x[2] - s.c[2];
         return a * a + b * b + c * c <= s.r * s.r;
     }
+    if (s.kind == 2) {   // cylinder along z: radius r, height s[2]
+        double a = x[0] - s.c[0], b = x[1] - s.c[1];
+        return a * a + b * b <= s.r * s.r && fabs(x[2] - s.c[2]) <= 0.5 * s.s[2];
+    }
     for (int a = 0; a < 3; ++a)
         if (!(fabs(x[a] - s.c[a]) <= 0.5 * s.s[a])) return false;
     return true;
@@ -42,6 +46,26 @@ __device__ inline void surfpt(const DShape &s, const double x[3], double r0[3], 
         if (nd == 0.0) { n[0] = 1; n[1] = 0; n[2] = 0; }
         else { n[0] = d[0] / nd; n[1] = d[1] / nd; n[2] = d[2] / nd; }
         for (int a = 0; a < 3; ++a) r0[a] = s.c[a] + s.r * n[a];
+        return;
+    }
+    if (s.kind == 2) {   // the nearer of the lateral surface and an end cap
+        double d0 = x[0] - s.c[0], d1 = x[1] - s.c[1], dz = x[2] - s.c[2];
+        double nd = sqrt(d0 * d0 + d1 * d1);
+        if (nd - s.r >= fabs(dz) - 0.5 * s.s[2]) {
+            if (nd == 0.0) { n[0] = 1; n[1] = 0; }
+            else { n[0] = d0 / nd; n[1] = d1 / nd; }
+            n[2] = 0;
+            r0[0] = s.c[0] + s.r * n[0];
+            r0[1] = s.c[1] + s.r * n[1];
+            r0[2] = x[2];
+        } else {
+            double sg = dz < 0 ? -1.0 : 1.0;
+            n[0] = n[1] = 0;
+            n[2] = sg;
+            r0[0] = x[0];
+            r0[1] = x[1];
+            r0[2] = s.c[2] + sg * 0.5 * s.s[2];
+        }
         return;
     }
     int ax = 0;
@@ -241,12 +265,15 @@ int matparams(mfd_ctx *ctx, const mfd_grid *g, int64_t nobj, const mfd_shape *sh
         const mfd_shape &a = shapes[o];
         DShape &d = ds[o];
         d.kind = a.kind;
-        if (a.kind != 0 && a.kind != 1) { set_error("matparams: shape kind must be 0 (ball) or 1 (cuboid)"); return MFD_ERR_ARG; }
+        if (a.kind != 0 && a.kind != 1 && a.kind != 2) {
+            set_error("matparams: shape kind must be 0 (ball), 1 (cuboid) or 2 (cylinder along z)");
+            return MFD_ERR_ARG;
+        }
         for (int k = 0; k < 3; ++k) {
             d.c[k] = a.c[k];
             d.s[k] = a.s[k];
             // conservative box: the exact test below decides
-            double half = a.kind == 0 ? a.r : 0.5 * a.s[k];
+            double half = a.kind == 0 ? a.r : (a.kind == 2 && k < 2 ? a.r : 0.5 * a.s[k]);
             double pad = 1e-9 * (fabs(a.c[k]) + fabs(half)) + 1e-300;
             d.lo[k] = a.c[k] - half - pad;
             d.hi[k] = a.c[k] + half + pad;

@@ -80,6 +80,14 @@ class Cuboid:
     s: Tuple[float, float, float]
 
 
+@dataclass
+class Cylinder:
+    """Cylinder along z: centre c, radius r, height h (a 2D disk extruded along z)."""
+    c: Tuple[float, float, float]
+    r: float
+    h: float = 1e30
+
+
 Sphere, Box = Ball, Cuboid
 
 
@@ -245,8 +253,8 @@ def add_obj(mdl, matname, *shapes, ε=1.0, μ=1.0):
         shapes = tuple(shapes[0])
     mat = Material(matname, _tensor(ε), _tensor(μ))
     for s in shapes:
-        if not isinstance(s, (Ball, Cuboid)):
-            raise TypeError("shapes must be Ball or Cuboid")
+        if not isinstance(s, (Ball, Cuboid, Cylinder)):
+            raise TypeError("shapes must be Ball, Cuboid or Cylinder")
         mdl.objs.append(Object(s, mat))
 
 
@@ -351,6 +359,10 @@ def _shapes_tables(mdl):
         if isinstance(s, Ball):
             arr[i].kind = 0
             arr[i].r = float(s.r)
+        elif isinstance(s, Cylinder):
+            arr[i].kind = 2
+            arr[i].r = float(s.r)
+            arr[i].s[2] = float(s.h)
         else:
             arr[i].kind = 1
             for a in range(3):

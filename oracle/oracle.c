@@ -635,6 +635,10 @@ static int inside(const og_shape *s, const double x[3]) {
         double a = x[0] - s->c[0], b = x[1] - s->c[1], c = x[2] - s->c[2];
         return a * a + b * b + c * c <= s->r * s->r;
     }
+    if (s->kind == 2) {   /* cylinder along z, radius r, height s[2] */
+        double a = x[0] - s->c[0], b = x[1] - s->c[1];
+        return a * a + b * b <= s->r * s->r && fabs(x[2] - s->c[2]) <= 0.5 * s->s[2];
+    }
     for (int a = 0; a < 3; ++a) if (!(fabs(x[a] - s->c[a]) <= 0.5 * s->s[a])) return 0;
     return 1;
 }
@@ -647,6 +651,20 @@ static void surfpt(const og_shape *s, const double x[3], double r0[3], double n[
         if (nd == 0.0) { n[0] = 1; n[1] = 0; n[2] = 0; }
         else { n[0] = d[0] / nd; n[1] = d[1] / nd; n[2] = d[2] / nd; }
         for (int a = 0; a < 3; ++a) r0[a] = s->c[a] + s->r * n[a];
+        return;
+    }
+    if (s->kind == 2) {   /* the nearer of the lateral surface and an end cap */
+        double d0 = x[0] - s->c[0], d1 = x[1] - s->c[1], dz = x[2] - s->c[2];
+        double nd = sqrt(d0 * d0 + d1 * d1);
+        if (nd - s->r >= fabs(dz) - 0.5 * s->s[2]) {
+            if (nd == 0.0) { n[0] = 1; n[1] = 0; } else { n[0] = d0 / nd; n[1] = d1 / nd; }
+            n[2] = 0;
+            r0[0] = s->c[0] + s->r * n[0]; r0[1] = s->c[1] + s->r * n[1]; r0[2] = x[2];
+        } else {
+            double sg = dz < 0 ? -1.0 : 1.0;
+            n[0] = n[1] = 0; n[2] = sg;
+            r0[0] = x[0]; r0[1] = x[1]; r0[2] = s->c[2] + sg * 0.5 * s->s[2];
+        }
         return;
     }
     int ax = 0; double best = -INFINITY;

@@ -40,7 +40,7 @@ typedef struct {
 } og_desc;
 
 typedef struct {
-    int32_t kind;             /* 0 = ball, 1 = axis-aligned cuboid */
+    int32_t kind;             /* 0 = ball, 1 = axis-aligned cuboid, 2 = cylinder along z (c, r, height s[2]) */
     int32_t pad;
     double c[3];              /* centre */
     double r;                 /* ball radius */

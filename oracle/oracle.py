@@ -205,7 +205,8 @@ def create_b(spec, omega, mu_arr, je, jm, ft=0, eps_arr=None):
 
 
 def matparams(spec, shapes, eps_t, mu_t):
-    """shapes: list of ('ball', c, r) / ('cuboid', c, s); eps_t/mu_t: (nobj,3,3)."""
+    """shapes: list of ('ball', c, r) / ('cuboid', c, s) / ('cylinder', c, r, h);
+    eps_t/mu_t: (nobj,3,3)."""
     d = _desc(spec)
     N = _N(spec)
     arr = (Shape * max(len(shapes), 1))()
@@ -213,6 +214,10 @@ def matparams(spec, shapes, eps_t, mu_t):
         if s[0] == "ball":
             arr[i].kind = 0
             arr[i].r = float(s[2])
+        elif s[0] == "cylinder":
+            arr[i].kind = 2
+            arr[i].r = float(s[2])
+            arr[i].s[2] = float(s[3])
         else:
             arr[i].kind = 1
             for a in range(3):

@@ -27,7 +27,9 @@ def shapes_of(mdl):
     out, et, mt = [], [], []
     for o in mdl.objs:
         s = o.shape
-        if hasattr(s, "r"):
+        if hasattr(s, "h"):
+            out.append(("cylinder", tuple(s.c), s.r, s.h))
+        elif hasattr(s, "r"):
             out.append(("ball", tuple(s.c), s.r))
         else:
             out.append(("cuboid", tuple(s.c), tuple(s.s)))

@@ -192,6 +192,7 @@ def geom_model(pkg, N=(14, 12, 13), nonuni=True, seed=0):
         pkg.add_obj(m, f"box{k}", pkg.Cuboid(tuple(rng.uniform(3, 9, 3)), tuple(rng.uniform(1.5, 5, 3))),
                     ε=rng.uniform(2, 12))
     pkg.add_obj(m, "ball", pkg.Ball(tuple(c0), 3.7), ε=12.0 + 0.5j)
+    pkg.add_obj(m, "rod", pkg.Cylinder((c0[0] + 2.3, c0[1] - 1.9, c0[2]), 2.2, 6.5), ε=5.0)
     pkg.add_obj(m, "aniso", pkg.Ball((c0[0] + 2, c0[1] - 1, c0[2]), 1.9), ε=np.diag([3.0, 4.0, 5.0]), μ=1.0)
     return m
 
@@ -593,3 +594,68 @@ def test_adjoint_design_region(pkg, O):
         dof = c + 3 * (i + N[0] * (j + N[1] * k))
         say("  voxel", (i, j, k, c), "fd", fd, "adjoint", grad[dof])
         assert grad[dof] == pytest.approx(fd, rel=1e-5, abs=1e-9 * np.abs(grad).max())
+
+
+def model2d(pkg, kind, N=(14, 11), npml=2, bloch=(False, False), om=0.6):
+    T = pkg.twod
+    lp = nonuniform(N, 21)
+    m = (T.ModelTE if kind == "te" else T.ModelTM)(T.Grid2(lp, bloch))
+    T.set_ωpml(m, om)
+    T.set_Npml(m, ((npml, npml), (npml, npml)))
+    c = [0.5 * (a[0] + a[-1]) for a in lp]
+    T.add_obj(m, "bg", T.Rect(tuple(c), (lp[0][-1] - lp[0][0], lp[1][-1] - lp[1][0])), ε=1.0)
+    T.add_obj(m, "rod", T.Disk((c[0] + 0.7, c[1] - 0.4), 2.6), ε=9.0 + 0.2j)
+    T.add_obj(m, "bar", T.Rect((c[0] - 2.1, c[1] + 1.3), (1.7, 3.1)), ε=4.0)
+    T.calc_matparams(m)
+    return m
+
+
+@pytest.mark.parametrize("kind", ["te", "tm"])
+def test_model2d_operator_and_solve(kind, pkg, O):
+    """2D TE/TM (te.jl, tm.jl) as the single-z-cell extruded 3D model: the
+    polarisations decouple exactly, the 2D operator equals the oracle's
+    extruded operator restricted to the 2D DOFs, and the solve matches the
+    direct solve of the 2D CSC."""
+    T = pkg.twod
+    om = 0.6
+    m = model2d(pkg, kind, om=om)
+    s3 = spec_of(m.m3)
+    shapes, et, mt = shapes_of(m.m3)
+    ea, ma = O.matparams(s3, shapes, et, mt)
+    assert np.abs(m.m3.εarr - ea).max() <= 1e-12 * np.abs(ea).max()
+    A3 = O.to_scipy(O.create_A(s3, om, ea, ma)).tocsr()
+    te, tm = m.esel(), (T.ModelTM if kind == "te" else T.ModelTE)(m.grid).esel()
+    assert abs(A3[te][:, tm]).max() == 0 and abs(A3[tm][:, te]).max() == 0   # exact split
+    A2 = T.create_A(pkg.EE, om, m)
+    ref = A3[te][:, te]
+    assert abs(A2 - ref).max() <= 1e-13 * abs(ref).max()
+    T.add_srce(m, T.PointSrc2((7.1, 5.3), (1.0, 0.4) if kind == "te" else (1.0,)))
+    je, jm = T.create_srcs(m)
+    b = T.create_b(pkg.EE, om, m, je, jm)
+    op = T.MaxwellOperator2D(m, om)
+    say("model2d", kind)
+    e, st = op.solve(b, tol=1e-10)
+    xd = sla.spsolve(A2.tocsc(), b)
+    assert np.linalg.norm(e - xd) / np.linalg.norm(xd) <= 1e-7
+    assert np.linalg.norm(b - A2 @ e) / np.linalg.norm(b) <= 1.05e-10
+    op.close()
+
+
+def test_model2d_tm_is_five_point_laplacian(pkg):
+    """Uniform grid, no PML: the TM operator is −Δ₅ − ω² ε with the textbook
+    5-point stencil (diagonal 4 − ω²ε, four −1 neighbours in the interior)."""
+    T = pkg.twod
+    N = (9, 7)
+    m = T.ModelTM(T.Grid2([np.arange(n + 1.0) for n in N]))
+    T.add_obj(m, "bg", T.Rect((N[0] / 2, N[1] / 2), (float(N[0]), float(N[1]))), ε=2.5)
+    T.calc_matparams(m)
+    om = 0.7
+    A = T.create_A(pkg.EE, om, m).tocsr()
+    for i in range(1, N[0] - 1):
+        for j in range(1, N[1] - 1):
+            p = i + N[0] * j
+            row = A.getrow(p)
+            assert row[0, p] == pytest.approx(4 - om ** 2 * 2.5, abs=1e-13)
+            for q in (p - 1, p + 1, p - N[0], p + N[0]):
+                assert row[0, q] == pytest.approx(-1.0, abs=1e-13)
+            assert row.nnz == 5

@@ -149,3 +149,42 @@ def test_priority_and_mu(O):
     assert ea[3, 3, 3, 0, 0] == 3.0 or ea[2, 3, 3, 0, 0] == 3.0   # the last-added object wins
     assert ma[0, 0, 0, 0, 0] == 1.0
     assert np.any(ma[..., 0, 0] == 1.5)
+
+
+def test_cylinder_smoothing_bruteforce(O):
+    """A z-cylinder (2D disk extruded): smoothed ε_xx matches Kottke's closed
+    form with a supersampled in-plane fill fraction and the radial normal, is
+    the same on every z plane, and has no xz/yz coupling."""
+    N = (16, 16, 3)
+    s = spec(N)
+    R, e_in = 4.6, 10.0
+    c = np.array([8.3, 7.8, 1.5])
+    shapes = [("cuboid", (8, 8, 1.5), (16.0, 16.0, 3.0)), ("cylinder", tuple(c), R, 1e30)]
+    I3 = np.eye(3)
+    ea, ma = O.matparams(s, shapes, [I3, e_in * I3], [I3, I3])
+    for k in range(1, N[2]):
+        np.testing.assert_array_equal(ea[..., k, :, :], ea[..., 0, :, :])
+    assert np.all(ea[..., 0, 2] == 0) and np.all(ea[..., 1, 2] == 0)
+    g = (np.arange(24) + 0.5) / 24
+    sub = np.stack(np.meshgrid(g, g, indexing="ij"), -1).reshape(-1, 2)
+    checked = 0
+    for i in range(N[0]):
+        for j in range(N[1]):
+            got = ea[i, j, 0, 0, 0].real
+            x0 = np.array([i + 0.5, j])           # E_x location in the plane
+            vmin = np.array([i, j - 0.5])         # its voxel: primal x, dual y
+            X = vmin + sub
+            r = np.mean(np.sum((X - c[:2]) ** 2, axis=1) <= R * R)
+            if got in (1.0, e_in):
+                assert r < 0.35 or r > 0.65
+                continue
+            if not 0.05 < r < 0.95:
+                continue
+            n = (x0 - c[:2]) / np.linalg.norm(x0 - c[:2])
+            epar = r * e_in + (1 - r)
+            eperp = 1 / (r / e_in + (1 - r))
+            ref = epar + (eperp - epar) * n[0] ** 2
+            assert abs(got - ref) <= 0.06 * (e_in - 1), (i, j, got, ref, r)
+            checked += 1
+    assert checked > 20
+    assert ea[8, 8, 1, 2, 2] == e_in and ea[0, 0, 0, 2, 2] == 1.0
