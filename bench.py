@@ -57,12 +57,13 @@ def parse():
                     help="fixed stencil tiles 'TY,KZ,order/TY,KZ,order' (FP64/FP32) instead of the autotuner, "
                          "so that separate hardware-counter passes launch the same kernels")
     ap.add_argument("--workload", default="random-boxes",
-                    choices=["random-boxes", "batched-rhs", "uneven-batch", "adjoint"],
+                    choices=["random-boxes", "batched-rhs", "uneven-batch", "adjoint", "large-aniso"],
                     help="random-boxes: the metric line; the others time the batched-RHS, uneven-problem "
                          "and adjoint configurations")
     ap.add_argument("--nrhs", type=int, default=4, help="batched-rhs: right-hand sides")
     ap.add_argument("--count", type=int, default=32, help="uneven-batch: problems")
     ap.add_argument("--voxels", type=int, default=4, help="adjoint: finite-difference voxels")
+    ap.add_argument("--maxit", type=int, default=2000, help="large-aniso: solver steps")
     return ap.parse_args()
 
 
@@ -211,6 +212,48 @@ def run_uneven_batch(a):
             "iterations_sequential": its}
 
 
+def run_large_aniso(a):
+    """384×384×256 anisotropic case (200 smoothed spheres, full 3×3 ε, 8
+    dipoles): assembly on the device, apply bandwidth of the full-tensor
+    stencil, and a bounded batched FP64 solve (a.maxit steps) with the
+    residual it reaches and the time per step."""
+    import numpy as np
+    import torch
+    import maxwellfdfd as M
+    from maxwellfdfd import workloads as W
+    log = lambda *x: print("[large-aniso]", *x, file=sys.stderr, flush=True)
+    m, om, dips = W.random_spheres(ndip=a.nrhs)
+    t0 = time.perf_counter()
+    M.calc_matparams(m)
+    t_mat = time.perf_counter() - t0
+    full = bool(np.any(m.εarr[..., 0, 1] != 0))
+    log("matparams", round(t_mat, 2), "s; off-diagonal ε present:", full)
+    B = W.dipole_rhs(m, om, dips)
+    op = M.MaxwellOperator(m, om)
+    cells = m.grid.M
+    tiles = op.autotune(1, 5)
+    log("tiles", tiles)
+    ms_apply = op.bench_apply(1, "fp64", iters=10)
+    bpc = 6 * 16 + 6 * 16                   # x + y, and the six stored ε entries (3 diagonal + 3 corner)
+    log("apply", round(ms_apply, 3), "ms")
+    dev = torch.device("cuda:0")
+    b_d = torch.from_numpy(B).to(dev)
+    e_d = torch.zeros_like(b_d)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = op.solve_dev(b_d.data_ptr(), e_d.data_ptr(), a.nrhs, tol=1e-8, maxit=a.maxit, raise_on_noconv=False)
+    torch.cuda.synchronize()
+    t_solve = time.perf_counter() - t0
+    op.close()
+    return {"workload": f"large-aniso {m.grid.N[0]}x{m.grid.N[1]}x{m.grid.N[2]}, 200 smoothed spheres "
+                        f"(full eps tensors: {full}), {a.nrhs} dipoles, FP64 SQMR bounded at {a.maxit} steps",
+            "cells": int(cells), "matparams_s": round(t_mat, 3), "tiles": [list(tiles[0]), list(tiles[1])],
+            "apply_ms": round(ms_apply, 4),
+            "apply_GBps": round(bpc * cells / (ms_apply * 1e-3) / 1e9, 1),
+            "solve_s": round(t_solve, 3), "steps": int(st.iterations), "relres_reached": st.relres,
+            "converged": bool(st.converged), "ms_per_step_all_rhs": round(1e3 * t_solve / max(st.iterations, 1), 3)}
+
+
 def run_adjoint(a):
     """128×128×64 adjoint case: forward solve, adjoint solve and ∂f/∂ε, with
     central finite differences (h = 1e-4, warm-started solves) on seeded design voxels."""
@@ -268,6 +311,8 @@ def main():
             out = run_batched_rhs(a)
         elif a.workload == "uneven-batch":
             out = run_uneven_batch(a)
+        elif a.workload == "large-aniso":
+            out = run_large_aniso(a)
         else:
             out = run_adjoint(a)
         print(json.dumps(out))

@@ -572,9 +572,12 @@ __global__ void k_pack(const c64 *eps, const c64 *mu, int64_t M, c64 *epsd, c64 
     if (epso) {
         const int vv[3] = {0, 0, 1}, ww[3] = {1, 2, 2};
         for (int x = 0; x < 3; ++x) {
-            // ε_vw and ε_wv share one stored value: they may differ by rounding only
+            // ε_vw and ε_wv share one stored value: they may differ by rounding
+            // only, measured against the size of the tensor (its diagonal too)
             c64 a = eps[p + M * (vv[x] + 3 * ww[x])], b = eps[p + M * (ww[x] + 3 * vv[x])];
-            double sc = fmax(fabs(a.re) + fabs(a.im), fabs(b.re) + fabs(b.im));
+            c64 dv = eps[p + M * (vv[x] + 3 * vv[x])], dw = eps[p + M * (ww[x] + 3 * ww[x])];
+            double sc = fabs(a.re) + fabs(a.im) + fabs(b.re) + fabs(b.im) + fabs(dv.re) + fabs(dv.im) +
+                        fabs(dw.re) + fabs(dw.im);
             if (fabs(a.re - b.re) + fabs(a.im - b.im) > 1e-12 * sc) atomicOr(asym, 1);
             epso[3 * p + x] = mk<double>(0.5 * (a.re + b.re), 0.5 * (a.im + b.im));
         }
@@ -779,12 +782,17 @@ static void launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int
     int kz = std::is_same<R, double>::value ? op->tile64[1] : op->tile32[1];
     int xr = std::is_same<R, double>::value ? op->tile64[2] : op->tile32[2];
     if (MODE == MODE_P && op->gd && ty == 8) ty = 4;   // the grad-div tile keeps two more LDS planes
-    if (!op->full && !op->has_mu && ty > 0) {
+    if (!op->has_mu && ty > 0) {
         int nx = (g.N[0] + 63) / 64, ny = (g.N[1] + ty - 1) / ty;
         int nch = (g.N[2] + kz - 1) / kz;
         while ((int64_t)nx * ny * nch > NPART) { kz *= 2; nch = (g.N[2] + kz - 1) / kz; }
         dim3 grid(nx, ny, (unsigned)(nch * nrhs));
-        if (MODE == MODE_P && op->gd) {
+        if (op->full) {   // off-diagonal ε (grad-div is never enabled with it)
+            if (ty == 8) k_apply_tiled<R, MODE, EPI, 8, false, false, true><<<grid, 512, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, false, false, true><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, false, false, true><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else k_apply_tiled<R, MODE, EPI, 1, false, false, true><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+        } else if (MODE == MODE_P && op->gd) {
             if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, true><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
             else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, true><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
             else k_apply_tiled<R, MODE, EPI, 1, true><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
@@ -1097,7 +1105,7 @@ int op_bench_iteration_impl(mfd_op *op, int64_t nrhs, int prec, int iters, doubl
 // operator P on FP64 or complex-FP32 vectors) for each candidate (TY, KZ, tile
 // order) and keeps the fastest per precision.
 int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best32, double *ms_out) {
-    if (op->full || op->has_mu) {   // the direct kernel serves these operators
+    if (op->has_mu) {   // the direct kernel serves these operators
         op->tile64[0] = op->tile32[0] = 0;
         if (best64) best64[0] = best64[1] = best64[2] = 0;
         if (best32) best32[0] = best32[1] = best32[2] = 0;

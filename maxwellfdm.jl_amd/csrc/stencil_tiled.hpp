@@ -107,7 +107,10 @@ __device__ inline void plane_init(const DGrid<R> &g, int vk, PlaneZ<R> &z) {
 // r (rall) and p_old (xall); the owner of each point also stores p (pnall) and
 // applies the previous step's deferred iterate update it += α_prev p_old
 // (itall), with β and α_prev taken from the solver scalars.
-template <typename R, int MODE, int EPI, int TY, bool GD, bool FUSE = false>
+// OFFD: the ε tensor has corner off-diagonals (xy, xz, yz); their term is
+// formed from the staged planes k−1 (three X_z values kept in registers),
+// k and k+1, with the corner ε read from global memory.
+template <typename R, int MODE, int EPI, int TY, bool GD, bool FUSE = false, bool OFFD = false>
 __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData<double> od, c64 om2,
                                                          const cx<R> *__restrict__ xall,
                                                          cx<R> *__restrict__ yall, const c64 *__restrict__ ball,
@@ -325,6 +328,7 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
 #pragma unroll
     for (int c = 0; c < 3; ++c) { xc[c] = xn[c] = czero<double>(); qc[c] = qn[c] = ec[c] = en[c] = cone<double>(); }
     c64 hpx = czero<double>(), hpy = czero<double>();
+    c64 zkm0 = czero<double>(), zkmx = czero<double>(), zkmy = czero<double>();
     if (!skip && k0 < k1) {
         stage(0, k0 - 1, xn, qn, en);
         stage(1, k0, xc, qc, ec);
@@ -340,6 +344,11 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
             hpy = az * (Xs[1][0][hj][hi] - X0x) - ax * (Xs[0][2][hj][hi + 1] - X0z);
         }
         if (GD) pplane(1, 0, k0, k0 & 1);
+        if (OFFD && own) {   // X_z of plane k0−1 at (i, j), (i+1, j), (i, j+1)
+            zkm0 = Xs[0][2][ty + 1][tx + 1];
+            zkmx = Xs[0][2][ty + 1][tx + 2];
+            zkmy = Xs[0][2][ty + 2][tx + 1];
+        }
         for (int k = k0; k < k1; ++k) {
             const int bk = (k - k0 + 1) & 1, bn = bk ^ 1;
             __syncthreads();   // buffer bn, Hs and Ps[(k+1)&1] are free
@@ -372,11 +381,60 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
                 }
                 int64_t p = i + (int64_t)g.N[0] * j + NxNy * k;
                 c64 tzm = axis_term(g, 2, k);
+                c64 term[3] = {czero<double>(), czero<double>(), czero<double>()};
+                if (OFFD) {
+                    // ∆l weights of the corner averages along each axis (reloaded
+                    // per plane from the cached 1D tables: fewer live registers)
+                    const c64 h5 = mk<double>(0.5, 0);
+                    const int im = i > 0 ? i - 1 : g.N[0] - 1, jm = j > 0 ? j - 1 : g.N[1] - 1;
+                    const int km = k > 0 ? k - 1 : g.N[2] - 1;
+                    const c64 ix = g.sdlinv[PRIM][0][i], iy = g.sdlinv[PRIM][1][j], iz = g.sdlinv[PRIM][2][k];
+                    const c64 wx0 = (h5 * g.sdl[DUAL][0][i]) * ix, wx1 = (h5 * g.sdl[DUAL][0][im]) * ix;
+                    const c64 wy0 = (h5 * g.sdl[DUAL][1][j]) * iy, wy1 = (h5 * g.sdl[DUAL][1][jm]) * iy;
+                    const c64 wz0 = (h5 * g.sdl[DUAL][2][k]) * iz, wz1 = (h5 * g.sdl[DUAL][2][km]) * iz;
+                    const int hi = tx + 1, hj = ty + 1;
+                    // corner cells p + û (wrapped), or -1 past a non-Bloch end
+                    int64_t cpx = i + 1 < g.N[0] ? p + 1 : (g.bloch[0] ? p - (g.N[0] - 1) : -1);
+                    int64_t cpy = j + 1 < g.N[1] ? p + g.N[0] : (g.bloch[1] ? p - (int64_t)(g.N[1] - 1) * g.N[0] : -1);
+                    int64_t cpz = k + 1 < g.N[2] ? p + NxNy : (g.bloch[2] ? p - (int64_t)(g.N[2] - 1) * NxNy : -1);
+                    // u = x: ww = y (xy) and ww = z (xz), corners p and p + x̂
+                    {
+                        c64 s = (h5 * od.epso[3 * p + 0]) * (wy0 * Xs[bk][1][hj][hi] + wy1 * Xs[bk][1][hj - 1][hi]) +
+                                (h5 * od.epso[3 * p + 1]) * (wz0 * Xs[bk][2][hj][hi] + wz1 * zkm0);
+                        if (cpx >= 0)
+                            s = s + (h5 * od.epso[3 * cpx + 0]) * (wy0 * Xs[bk][1][hj][hi + 1] + wy1 * Xs[bk][1][hj - 1][hi + 1]) +
+                                (h5 * od.epso[3 * cpx + 1]) * (wz0 * Xs[bk][2][hj][hi + 1] + wz1 * zkmx);
+                        term[0] = s;
+                    }
+                    // u = y: ww = x (xy) and ww = z (yz), corners p and p + ŷ
+                    {
+                        c64 s = (h5 * od.epso[3 * p + 0]) * (wx0 * Xs[bk][0][hj][hi] + wx1 * Xs[bk][0][hj][hi - 1]) +
+                                (h5 * od.epso[3 * p + 2]) * (wz0 * Xs[bk][2][hj][hi] + wz1 * zkm0);
+                        if (cpy >= 0)
+                            s = s + (h5 * od.epso[3 * cpy + 0]) * (wx0 * Xs[bk][0][hj + 1][hi] + wx1 * Xs[bk][0][hj + 1][hi - 1]) +
+                                (h5 * od.epso[3 * cpy + 2]) * (wz0 * Xs[bk][2][hj + 1][hi] + wz1 * zkmy);
+                        term[1] = s;
+                    }
+                    // u = z: ww = x (xz) and ww = y (yz), corners p and p + ẑ
+                    {
+                        c64 s = (h5 * od.epso[3 * p + 1]) * (wx0 * Xs[bk][0][hj][hi] + wx1 * Xs[bk][0][hj][hi - 1]) +
+                                (h5 * od.epso[3 * p + 2]) * (wy0 * Xs[bk][1][hj][hi] + wy1 * Xs[bk][1][hj - 1][hi]);
+                        if (cpz >= 0)
+                            s = s + (h5 * od.epso[3 * cpz + 1]) * (wx0 * Xs[bn][0][hj][hi] + wx1 * Xs[bn][0][hj][hi - 1]) +
+                                (h5 * od.epso[3 * cpz + 2]) * (wy0 * Xs[bn][1][hj][hi] + wy1 * Xs[bn][1][hj - 1][hi]);
+                        term[2] = s;
+                    }
+                    // X_z of this plane, for the next one
+                    zkm0 = Xs[bk][2][hj][hi];
+                    zkmx = Xs[bk][2][hj][hi + 1];
+                    zkmy = Xs[bk][2][hj + 1][hi];
+                }
 #pragma unroll
                 for (int u = 0; u < 3; ++u) {
                     c64 out;
                     if (MODE == MODE_A) {
                         out = cc[u] - om2 * (ec[u] * xc[u]);
+                        if (OFFD) out = out - om2 * term[u];
                     } else {
                         c64 lf = qc[u] * (Txy[u] * g.ssq[u == 2 ? DUAL : PRIM][2][k]);
                         if (GD) {
@@ -385,6 +443,7 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
                             c64 cd = u == 0 ? tmy + tzm : (u == 1 ? tzm + tmx : tmx + tmy);
                             out = lf * cc[u] + (cone<double>() - cd * (qc[u] * qc[u])) * xc[u];
                         }
+                        if (OFFD) out = out - om2 * (lf * term[u]);
                     }
                     int64_t e = dofidx(g, u, p);
                     if (EPI == EPI_RESID) {

@@ -507,7 +507,7 @@ class MaxwellOperator:
         check(lib().mfd_op_set_graddiv(self._h, int(bool(enable))))
 
     def set_tile(self, precision, ty, kz, xcd_order=1):
-        """Stencil tile shape: TY rows per block (1, 2, 4; 0 = direct kernel), KZ planes per
+        """Stencil tile shape: TY rows per block (1, 2, 4, 8; 0 = direct kernel), KZ planes per
         block, and tile order (1: a contiguous run of tiles per XCD, 0: dispatch order)."""
         check(lib().mfd_op_set_tile(self._h, L.MFD_FP64 if precision == "fp64" else L.MFD_MIXED, ty, kz,
                                     xcd_order))
@@ -565,10 +565,12 @@ class MaxwellOperator:
         return e, st
 
     def solve_dev(self, b_ptr, e_ptr, nrhs=1, tol=1e-8, precision="fp64", maxit=None, inner_tol=None,
-                  check_every=None, use_graph=True, method=None):
+                  check_every=None, use_graph=True, method=None, raise_on_noconv=True):
         o = self._opts(tol, precision, maxit, inner_tol, check_every, use_graph, method)
         st = L.mfd_solve_stats()
-        check(lib().mfd_solve_dev(self._h, C.c_void_p(b_ptr), C.c_void_p(e_ptr), nrhs, C.byref(o), C.byref(st)))
+        rc = lib().mfd_solve_dev(self._h, C.c_void_p(b_ptr), C.c_void_p(e_ptr), nrhs, C.byref(o), C.byref(st))
+        if rc != L.MFD_OK and (raise_on_noconv or rc != L.MFD_ERR_NOCONV):
+            check(rc)
         return st
 
     def solve_batch_dev(self, others, b_ptrs, e_ptrs, tol=1e-8, precision="fp64", maxit=None, inner_tol=None,

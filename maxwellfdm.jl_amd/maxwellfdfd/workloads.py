@@ -115,3 +115,24 @@ def adjoint_design(N=(128, 128, 64), npml=10, design=(64, 64, 16), lam=24.0, see
     mon = [2 + 3 * (i + N[0] * (j + N[1] * k)) for k in range(zmon, zmon + monitor[2])
            for j in range(j0, j0 + monitor[1]) for i in range(i0, i0 + monitor[0])]
     return m, om, b, np.array(mon), sl
+
+
+def random_spheres(N=(384, 384, 256), npml=12, nsph=200, rmin=4.0, rmax=20.0, emin=2.0, emax=12.0, lam=32.0,
+                   seed=5, ndip=8):
+    """Large anisotropic case: background ε=1 with nsph seeded spheres
+    (radius ~ U[rmin, rmax], ε ~ U[emin, emax], centres inside the non-PML
+    interior); subpixel smoothing then yields full 3×3 ε tensors at the
+    sphere surfaces.  Returns (model, ω, dipoles)."""
+    om = 2 * np.pi / lam
+    m = base_model(N, npml, om)
+    rng = np.random.default_rng(seed)
+    for k in range(nsph):
+        r = rng.uniform(rmin, rmax)
+        c = tuple(rng.uniform(npml + r, n - npml - r) for n in N)
+        M.add_obj(m, f"sphere{k}", M.Ball(c, r), ε=rng.uniform(emin, emax))
+    dips = []
+    for _ in range(ndip):
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        dips.append((np.array([rng.uniform(npml + 2, n - npml - 2) for n in N]), v))
+    return m, om, dips
