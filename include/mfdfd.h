@@ -122,6 +122,15 @@ int mfd_curl_nnz(mfd_ctx *ctx, const mfd_grid *g, int which, int64_t *nnz);
 int mfd_curl_csc(mfd_ctx *ctx, const mfd_grid *g, int which, int64_t *colptr, int64_t *rowval,
                  mfd_c64 *nzval);
 
+/* create_Mcs (model.jl:287-306): which 0 = Mcₑ, 1 = Mcₘ, the operators that
+ * interpolate the solution fields to the voxel corners (MaxwellBase
+ * create_mean with isfwd = boundft .!= EE / .!= HH): F_w goes to the w slot
+ * of its corners by the ∆l-weighted average along w that Pε/Pμ's off-diagonal
+ * blocks also use.  Same two phases as the curls. */
+int mfd_Mcs_nnz(mfd_ctx *ctx, const mfd_grid *g, int which, int64_t *nnz);
+int mfd_Mcs_csc(mfd_ctx *ctx, const mfd_grid *g, int which, int64_t *colptr, int64_t *rowval,
+                mfd_c64 *nzval);
+
 /* calc_matparams! (full.jl:16-70): assignment + Kottke subpixel smoothing.
  * eps_t/mu_t: nobj 3x3 column-major tensors; objects later in the list win. */
 int mfd_matparams(mfd_ctx *ctx, const mfd_grid *g, int64_t nobj, const mfd_shape *shapes,

@@ -101,16 +101,12 @@ __device__ void curl_col(const DGrid<double> &g, int which, int w, int64_t p, EL
     L.sort_merge();
 }
 
-// Column (w, q) of Pε (which = 0) or Pμ (which = 1).  The off-diagonal block
-// (v, w) takes F_w to the corners of grid g0 with a ∆l-weighted average along
-// w (direction isfwd_in[w]; weights s∆l at the field planes over 2 s∆l at the
-// corner planes), multiplies by parr(v,w) at the corner, and takes the corner
-// values to F_v by a half-half average along v in the other direction.
-__device__ void param_col(const DGrid<double> &g, int which, const c64 *parr, int full, int w, int64_t qc,
-                          EList<9> &L) {
-    int64_t M = g.M;
-    L.push(dofidx(g, w, qc), 0, parr[qc + M * (w + 3 * w)]);
-    if (!full) return;
+// Corners that the field value F_w(qc) contributes to, with their weights:
+// the ∆l-weighted average along w (direction isfwd_in[w]; weights s∆l at the
+// field planes over 2 s∆l at the corner planes) that create_mean performs for
+// Pε/Pμ and for create_Mcs (model.jl:287-306).  Returns the entry count (1 or 2).
+__device__ inline int corner_entries(const DGrid<double> &g, int which, int w, int64_t qc, int64_t cor[2],
+                                     c64 wt[2]) {
     int ft = which == 0 ? EE : HH;
     int q[3];
     unc3(g, qc, q);
@@ -118,6 +114,7 @@ __device__ void param_col(const DGrid<double> &g, int which, const c64 *parr, in
     int fwd_in_w = which == 0 ? (g.bft[w] != EE) : (g.bft[w] != HH);
     int Nw = g.N[w];
     int64_t stw = stride3(g, w);
+    int n = 0;
     for (int ic = 0; ic < 2; ++ic) {
         int64_t c = qc;
         int cw = q[w], fi = q[w];
@@ -133,7 +130,39 @@ __device__ void param_col(const DGrid<double> &g, int which, const c64 *parr, in
                 else continue;
             }
         }
-        c64 wi = ((0.5 * g.sdl[gf][w][fi]) * g.sdlinv[gc][w][cw]) * phin;
+        cor[n] = c;
+        wt[n] = ((0.5 * g.sdl[gf][w][fi]) * g.sdlinv[gc][w][cw]) * phin;
+        ++n;
+    }
+    return n;
+}
+
+// Column (w, q) of the corner-mean operator Mcₑ (which = 0) or Mcₘ (which = 1):
+// F_w at q goes to the w-component slot of its corners.
+__device__ void mean_col(const DGrid<double> &g, int which, int w, int64_t qc, EList<2> &L) {
+    int64_t cor[2];
+    c64 wt[2];
+    int n = corner_entries(g, which, w, qc, cor, wt);
+    for (int x = 0; x < n; ++x) L.push(dofidx(g, w, cor[x]), x, wt[x]);
+    L.sort_merge();
+}
+
+// Column (w, q) of Pε (which = 0) or Pμ (which = 1).  The off-diagonal block
+// (v, w) takes F_w to the corners of grid g0 with a ∆l-weighted average along
+// w (direction isfwd_in[w]; weights s∆l at the field planes over 2 s∆l at the
+// corner planes), multiplies by parr(v,w) at the corner, and takes the corner
+// values to F_v by a half-half average along v in the other direction.
+__device__ void param_col(const DGrid<double> &g, int which, const c64 *parr, int full, int w, int64_t qc,
+                          EList<9> &L) {
+    int64_t M = g.M;
+    L.push(dofidx(g, w, qc), 0, parr[qc + M * (w + 3 * w)]);
+    if (!full) return;
+    int64_t cor[2];
+    c64 wt[2];
+    const int ncor = corner_entries(g, which, w, qc, cor, wt);
+    for (int ic = 0; ic < ncor; ++ic) {
+        const int64_t c = cor[ic];
+        const c64 wi = wt[ic];
         int cq[3];
         unc3(g, c, cq);
         for (int v = 0; v < 3; ++v) {
@@ -173,6 +202,21 @@ __global__ void k_curl(DGrid<double> g, int which, int64_t n, int64_t *cnt, cons
     col2cp(g, col, w, p);
     EList<4> L;
     curl_col(g, which, w, p, L);
+    if (!FILL) { cnt[col] = L.n; return; }
+    int64_t o = colptr[col];
+    for (int x = 0; x < L.n; ++x) { rowval[o + x] = L.e[x].row + 1; nzval[o + x] = L.e[x].v; }
+}
+
+template <bool FILL>
+__global__ void k_mean(DGrid<double> g, int which, int64_t n, int64_t *cnt, const int64_t *colptr,
+                       int64_t *rowval, c64 *nzval) {
+    int64_t col = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (col >= n) return;
+    int w;
+    int64_t p;
+    col2cp(g, col, w, p);
+    EList<2> L;
+    mean_col(g, which, w, p, L);
     if (!FILL) { cnt[col] = L.n; return; }
     int64_t o = colptr[col];
     for (int x = 0; x < L.n; ++x) { rowval[o + x] = L.e[x].row + 1; nzval[o + x] = L.e[x].v; }
@@ -330,6 +374,16 @@ int curl_csc(mfd_ctx *ctx, const DevGrid &G, int which, int64_t *nnz, int64_t *c
     return csc_drive(s, n, nnz, colptr, rowval, nzval,
                      [&](int64_t *cnt) { k_curl<false><<<nblk(n, 256), 256, 0, s>>>(g, which, n, cnt, nullptr, nullptr, nullptr); },
                      [&](int64_t *cp, int64_t *rv, c64 *nz) { k_curl<true><<<nblk(n, 256), 256, 0, s>>>(g, which, n, nullptr, cp, rv, nz); });
+}
+
+int mean_csc(mfd_ctx *ctx, const DevGrid &G, int which, int64_t *nnz, int64_t *colptr, int64_t *rowval,
+             mfd_c64 *nzval) {
+    const DGrid<double> &g = G.d64;
+    int64_t n = 3 * g.M;
+    hipStream_t s = ctx->stream;
+    return csc_drive(s, n, nnz, colptr, rowval, nzval,
+                     [&](int64_t *cnt) { k_mean<false><<<nblk(n, 256), 256, 0, s>>>(g, which, n, cnt, nullptr, nullptr, nullptr); },
+                     [&](int64_t *cp, int64_t *rv, c64 *nz) { k_mean<true><<<nblk(n, 256), 256, 0, s>>>(g, which, n, nullptr, cp, rv, nz); });
 }
 
 DevArray::~DevArray() { if (p) hipFree(p); }

@@ -16,6 +16,9 @@ int upload_param(mfd_ctx *ctx, const DevGrid &G, const mfd_c64 *parr, DevArray &
 // colptr == nullptr: only *nnz is computed.
 int curl_csc(mfd_ctx *ctx, const DevGrid &G, int which, int64_t *nnz, int64_t *colptr, int64_t *rowval,
              mfd_c64 *nzval);
+// corner-mean operator of create_Mcs: which 0 = Mcₑ, 1 = Mcₘ
+int mean_csc(mfd_ctx *ctx, const DevGrid &G, int which, int64_t *nnz, int64_t *colptr, int64_t *rowval,
+             mfd_c64 *nzval);
 int param_csc(mfd_ctx *ctx, const DevGrid &G, int which, const mfd_c64 *parr, int64_t *nnz, int64_t *colptr,
               int64_t *rowval, mfd_c64 *nzval);
 // ft = EE: A = Cₘ (Pμ \ Cₑ) − ω² Pε; ft = HH: A = Cₑ (Pε \ Cₘ) − ω² Pμ

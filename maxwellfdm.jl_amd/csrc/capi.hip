@@ -205,6 +205,24 @@ int mfd_curl_csc(mfd_ctx *ctx, const mfd_grid *g, int which, int64_t *colptr, in
     return curl_csc(ctx, G, which, &nnz, colptr, rowval, nzval);
 }
 
+int mfd_Mcs_nnz(mfd_ctx *ctx, const mfd_grid *g, int which, int64_t *nnz) {
+    CHECK_CTX(ctx);
+    if (which != 0 && which != 1) { set_error("which must be 0 (Mcₑ) or 1 (Mcₘ)"); return MFD_ERR_ARG; }
+    DevGrid G;
+    MFD_TRY(G.create(g, ctx->stream));
+    return mean_csc(ctx, G, which, nnz, nullptr, nullptr, nullptr);
+}
+
+int mfd_Mcs_csc(mfd_ctx *ctx, const mfd_grid *g, int which, int64_t *colptr, int64_t *rowval, mfd_c64 *nzval) {
+    CHECK_CTX(ctx);
+    if (which != 0 && which != 1) { set_error("which must be 0 (Mcₑ) or 1 (Mcₘ)"); return MFD_ERR_ARG; }
+    if (!colptr || !rowval || !nzval) { set_error("null output buffer"); return MFD_ERR_ARG; }
+    DevGrid G;
+    MFD_TRY(G.create(g, ctx->stream));
+    int64_t nnz;
+    return mean_csc(ctx, G, which, &nnz, colptr, rowval, nzval);
+}
+
 int mfd_matparams(mfd_ctx *ctx, const mfd_grid *g, int64_t nobj, const mfd_shape *shapes, const mfd_c64 *eps_t,
                   const mfd_c64 *mu_t, mfd_c64 *eps_arr, mfd_c64 *mu_arr) {
     CHECK_CTX(ctx);

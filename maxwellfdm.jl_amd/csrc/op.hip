@@ -598,6 +598,15 @@ __global__ void k_jacobi(DGrid<double> g, OpData<double> od, c64 *q, int use) {
     }
 }
 
+// qs = q·T⁻¹ per E DOF: the tiled stencil scales its staged input with this one
+// factor and recovers q = qs·T for the owner's output
+__global__ void k_qscale(DGrid<double> g, const c64 *q, c64 *qs) {
+    int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= g.M) return;
+    int i, j, k;
+    cell3(g, p, i, j, k);
+    for (int c = 0; c < 3; ++c) qs[3 * p + c] = q[3 * p + c] * tfac(g, c, i, j, k, true);
+}
 
 // grad-div weight W = 1/ε̄² at each primal node, ε̄ the mean of the ε_ww of
 // the (up to 6) E edges meeting at the node
@@ -883,13 +892,14 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
     // device arrays: epsd, epso, muinv, q (FP64) and their FP32 copies, + asym flag
     size_t n3 = 3 * (size_t)M;
     size_t b64 = sizeof(c64) * n3, b32 = sizeof(c32) * n3;
-    size_t bytes = 4 * (b64 + b32) + (b64 + b32) / 3 + 512;
+    size_t bytes = 4 * (b64 + b32) + b64 + (b64 + b32) / 3 + 512;
     if (hipMalloc(&op->buf, bytes) != hipSuccess) { set_error("hipMalloc failed for operator arrays"); return bail(MFD_ERR_ALLOC); }
     char *p = (char *)op->buf;
     c64 *epsd = (c64 *)p; p += b64;
     c64 *epso = (c64 *)p; p += b64;
     c64 *muinv = (c64 *)p; p += b64;
     c64 *q = (c64 *)p; p += b64;
+    c64 *qs = (c64 *)p; p += b64;
     c32 *epsd32 = (c32 *)p; p += b32;
     c32 *epso32 = (c32 *)p; p += b32;
     c32 *muinv32 = (c32 *)p; p += b32;
@@ -909,6 +919,7 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
     op->d64.epso = full ? epso : nullptr;
     op->d64.muinv = op->has_mu ? muinv : nullptr;
     op->d64.q = q;
+    op->d64.qs = qs;
     op->d64.om2 = om2;
     op->d64.wnode = wn;
     // with a complex Bloch phase e^{-ikL} the operator is not complex-symmetric
@@ -921,6 +932,7 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
     if (op->gd) k_jacobi_gd<<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q);
     else if (op->has_mu) k_jacobi<true><<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q, 1);
     else k_jacobi<false><<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q, 1);
+    k_qscale<<<nb(M), BS, 0, s>>>(op->G.d64, q, qs);
     k_down<<<nb(n3), BS, 0, s>>>(epsd, epsd32, n3);
     if (full) k_down<<<nb(n3), BS, 0, s>>>(epso, epso32, n3);
     if (op->has_mu) k_down<<<nb(n3), BS, 0, s>>>(muinv, muinv32, n3);
@@ -929,6 +941,7 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
     op->d32.epso = full ? epso32 : nullptr;
     op->d32.muinv = op->has_mu ? muinv32 : nullptr;
     op->d32.q = q32;
+    op->d32.qs = nullptr;   // the tiled stencil reads the FP64 operator data only
     op->d32.wnode = wn32;
     op->d32.om2 = mk<float>((float)om2.re, (float)om2.im);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) { set_error("operator set-up failed"); return bail(MFD_ERR_HIP); }
@@ -961,6 +974,7 @@ int op_set_graddiv_impl(mfd_op *op, int enable) {
     if (op->gd) k_jacobi_gd<<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q);
     else if (op->has_mu) k_jacobi<true><<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q, 1);
     else k_jacobi<false><<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q, 1);
+    k_qscale<<<nb(M), BS, 0, s>>>(op->G.d64, q, (c64 *)op->d64.qs);
     k_down<<<nb(3 * M), BS, 0, s>>>(q, (c32 *)op->d32.q, 3 * M);
     op->ws.release();   // captured graphs hold the old kernels
     MFD_HIP(hipGetLastError());

@@ -26,6 +26,7 @@ struct OpData {
     const cx<R> *epso;    // [M][3] (xy, xz, yz) at primal corners, or null
     const cx<R> *muinv;   // [M][3] 1/μ_cc at H_c locations, or null
     const cx<R> *q;       // [M][3] Jacobi scaling diag(A)^-1/2 (of A' when the grad-div form is on)
+    const cx<R> *qs;      // [M][3] q·T⁻¹: the tiled stencil's input scaling of P = Q T A T⁻¹ Q
     const cx<R> *wnode;   // [M] grad-div weight W = 1/ε̄² at the primal nodes
     cx<R> om2;            // ω²
 };

@@ -66,13 +66,35 @@ __device__ inline cx<R> axis_term(const DGrid<R> &g, int a, int ia) {
     return ie * (g.sdlinv[DUAL][a][ia] + g.sdlinv[DUAL][a][im]);
 }
 
+// Read-only table entry at a block-uniform index, loaded through the constant
+// address space as one 16-byte value so that it compiles to a scalar load: its
+// wait (lgkmcnt) is independent of the vector-load counter the prefetch uses.
+typedef double d2v __attribute__((ext_vector_type(2)));
+__device__ inline c64 ldu(const c64 *tab, int idx) {
+    const int u = __builtin_amdgcn_readfirstlane(idx);
+    const d2v v = *(const __attribute__((address_space(4))) d2v *)(tab + u);
+    return mk<double>(v.x, v.y);
+}
+
+// axis_term along z at a block-uniform plane index, from scalar loads
+__device__ inline c64 axis_term_u(const DGrid<double> &g, int ia) {
+    const int Na = g.N[2];
+    c64 ie = ldu(g.sdlinv[PRIM][2], ia);
+    if (Na == 1 && g.bloch[2]) {
+        c64 f = mk<double>(2.0, 0.0) - g.ph[2] - g.phinv[2];
+        return (ie * ldu(g.sdlinv[DUAL][2], 0)) * f;
+    }
+    if (ia == 0 && !g.bloch[2]) return ie * ldu(g.sdlinv[DUAL][2], 0);
+    int im = ia > 0 ? ia - 1 : Na - 1;
+    return ie * (ldu(g.sdlinv[DUAL][2], ia) + ldu(g.sdlinv[DUAL][2], im));
+}
+
 // per-thread cached description of one staged point (fixed x-y position)
 template <typename R>
 struct StagePt {
     int gi, gj;        // wrapped indices
     bool ok;           // not a zero ghost / out of ring
     cx<R> ph;          // x-y wrap phase
-    cx<R> tinv[3];     // x·y part of T⁻¹ per component
 };
 
 template <typename R>
@@ -80,9 +102,6 @@ __device__ inline void stage_pt_init(const DGrid<R> &g, int vi, int vj, StagePt<
     s.ph = cone<R>();
     s.ok = wrap_ax(g, 0, vi, s.gi, s.ph) && wrap_ax(g, 1, vj, s.gj, s.ph);
     if (!s.ok) { s.gi = 0; s.gj = 0; }
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-        s.tinv[c] = g.ssqinv[c == 0 ? DUAL : PRIM][0][s.gi] * g.ssqinv[c == 1 ? DUAL : PRIM][1][s.gj];
 }
 
 // per-plane (block-uniform) z data
@@ -91,7 +110,6 @@ struct PlaneZ {
     int gk;
     bool ok;
     cx<R> ph;
-    cx<R> tinv[3];
 };
 
 template <typename R>
@@ -99,8 +117,6 @@ __device__ inline void plane_init(const DGrid<R> &g, int vk, PlaneZ<R> &z) {
     z.ph = cone<R>();
     z.ok = wrap_ax(g, 2, vk, z.gk, z.ph);
     if (!z.ok) z.gk = 0;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) z.tinv[c] = g.ssqinv[c == 2 ? DUAL : PRIM][2][z.gk];
 }
 
 // FUSE (COCG step): the operand is p = r + β p_old, formed while staging from
@@ -259,8 +275,8 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
                     } else xv = cvt<double>(x[dof]);
                     c64 qv = cone<double>();
                     if (MODE == MODE_P) {
-                        qv = cvt<double>(od.q[3 * p + c]);
-                        v = (xv * qv) * (P.tinv[c] * z.tinv[c]);
+                        qv = od.qs[3 * p + c];   // q·T⁻¹
+                        v = xv * qv;
                     } else v = xv;
                     c64 ev = cone<double>();
                     if (GD || (MODE == MODE_A && s == 0)) ev = cvt<double>(od.epsd[3 * p + c]);
@@ -275,19 +291,64 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
         }
     };
 
-    // H of plane vk (buffers bk: plane vk, bn: plane vk+1) for all H points
-    auto hplane = [&](int bk, int bn, int vk) {
+    // Software pipeline (no grad-div, fused or off-diagonal term, whose extra
+    // per-thread loads would wait on it): the global loads of plane k+2 are
+    // issued before plane k's H and output and land in registers while those
+    // run; commit() scales them into LDS one pass later.  Nothing between a
+    // fetch and its commit issues a per-thread global load.
+    constexpr bool PIPE = !GD && !FUSE && !OFFD;
+    constexpr int NPF = PIPE ? 1 + NRS : 1;
+    cx<R> pfx[NPF][3];
+    c64 pfq[MODE == MODE_P ? NPF : 1][3];
+    c64 pfe[3];
+    auto fetch = [&](int vk) {
         PlaneZ<double> z;
         plane_init(g, vk, z);
-        const c64 az = g.sdlinv[DUAL][2][z.gk];
-        const bool zok = vk >= 0 || g.bloch[2];
+#pragma unroll
+        for (int s = 0; s < NPF; ++s) {
+            if (!sact[s]) continue;
+            const bool ok = sp[s].ok && z.ok;
+            const int64_t p = sp[s].gi + (int64_t)g.N[0] * sp[s].gj + NxNy * z.gk;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                pfx[s][c] = ok ? x[dofidx(g, c, p)] : mk<R>((R)0, (R)0);
+                if (MODE == MODE_P) pfq[s][c] = ok ? od.qs[3 * p + c] : czero<double>();
+                if (MODE == MODE_A && s == 0) pfe[c] = ok ? od.epsd[3 * p + c] : cone<double>();
+            }
+        }
+    };
+    auto commit = [&](int b, int vk, c64 (&xr)[3], c64 (&qr)[3], c64 (&er)[3]) {
+        PlaneZ<double> z;
+        plane_init(g, vk, z);
+#pragma unroll
+        for (int s = 0; s < NPF; ++s) {
+            if (!sact[s]) continue;
+            const c64 ph = sp[s].ph * z.ph;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const c64 xv = cvt<double>(pfx[s][c]);
+                c64 v = xv;
+                if (MODE == MODE_P) v = xv * pfq[s][c];
+                if (s == 0) {
+                    xr[c] = xv;
+                    qr[c] = MODE == MODE_P ? pfq[s][c] : cone<double>();
+                    er[c] = MODE == MODE_A ? pfe[c] : cone<double>();
+                }
+                Xs[b][c][sjj[s]][sii[s]] = v * ph;
+            }
+        }
+    };
+
+    // H of an interior plane (buffers bk: that plane, bn: the next) for all H
+    // points; az is 1/s∆l_z of the plane
+    auto hplane = [&](int bk, int bn, c64 az) {
 #pragma unroll
         for (int r = 0; r < NHP; ++r) {
             int t = tid + r * C::NT;
             if (t >= C::HHt * C::HW) break;
             int hj = t / C::HW, hi = t - hj * C::HW;
             c64 h0 = czero<double>(), h1 = czero<double>(), h2 = czero<double>();
-            if (hok[r] && zok) {
+            if (hok[r]) {
                 c64 ax = hax[r], ay = hay[r];
                 c64 X0x = Xs[bk][0][hj][hi], X0y = Xs[bk][1][hj][hi], X0z = Xs[bk][2][hj][hi];
                 h0 = ay * (Xs[bk][2][hj + 1][hi] - X0z) - az * (Xs[bn][1][hj][hi] - X0y);
@@ -330,18 +391,32 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
     c64 hpx = czero<double>(), hpy = czero<double>();
     c64 zkm0 = czero<double>(), zkmx = czero<double>(), zkmy = czero<double>();
     if (!skip && k0 < k1) {
-        stage(0, k0 - 1, xn, qn, en);
-        stage(1, k0, xc, qc, ec);
+        // the owner's factors of H_x, H_y at plane k0−1 (loaded ahead of the
+        // prefetch, as every per-thread load in the pipelined loop is)
+        const bool hpv = own && (k0 > 0 || g.bloch[2]);
+        c64 pax = czero<double>(), pay = czero<double>(), paz = czero<double>();
+        if (hpv) {
+            paz = g.sdlinv[DUAL][2][k0 > 0 ? k0 - 1 : g.N[2] - 1];
+            pax = g.sdlinv[DUAL][0][i];
+            pay = g.sdlinv[DUAL][1][j];
+        }
+        if (PIPE) {
+            fetch(k0 - 1);
+            commit(0, k0 - 1, xn, qn, en);
+            fetch(k0);
+            commit(1, k0, xc, qc, ec);
+            fetch(k0 + 1);
+        } else {
+            stage(0, k0 - 1, xn, qn, en);
+            stage(1, k0, xc, qc, ec);
+        }
         __syncthreads();
         // H_x, H_y of plane k0−1 at the owned point (zero across a non-Bloch z ghost)
-        if (own && (k0 > 0 || g.bloch[2])) {
-            int gkm = k0 > 0 ? k0 - 1 : g.N[2] - 1;
-            c64 az = g.sdlinv[DUAL][2][gkm];
+        if (hpv) {
             int hi = tx + 1, hj = ty + 1;
             c64 X0x = Xs[0][0][hj][hi], X0y = Xs[0][1][hj][hi], X0z = Xs[0][2][hj][hi];
-            c64 ax = g.sdlinv[DUAL][0][i], ay = g.sdlinv[DUAL][1][j];
-            hpx = ay * (Xs[0][2][hj + 1][hi] - X0z) - az * (Xs[1][1][hj][hi] - X0y);
-            hpy = az * (Xs[1][0][hj][hi] - X0x) - ax * (Xs[0][2][hj][hi + 1] - X0z);
+            hpx = pay * (Xs[0][2][hj + 1][hi] - X0z) - paz * (Xs[1][1][hj][hi] - X0y);
+            hpy = paz * (Xs[1][0][hj][hi] - X0x) - pax * (Xs[0][2][hj][hi + 1] - X0z);
         }
         if (GD) pplane(1, 0, k0, k0 & 1);
         if (OFFD && own) {   // X_z of plane k0−1 at (i, j), (i+1, j), (i, j+1)
@@ -352,9 +427,18 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
         for (int k = k0; k < k1; ++k) {
             const int bk = (k - k0 + 1) & 1, bn = bk ^ 1;
             __syncthreads();   // buffer bn, Hs and Ps[(k+1)&1] are free
-            stage(bn, k + 1, xn, qn, en);
+            // z factors of plane k (scalar loads: waiting for them does not
+            // wait for the prefetch)
+            const c64 az_k = ldu(g.sdlinv[DUAL][2], k), ez_k = ldu(g.sdlinv[PRIM][2], k);
+            const c64 tzm = axis_term_u(g, k);
+            const c64 szD = ldu(g.ssq[DUAL][2], k), szP = ldu(g.ssq[PRIM][2], k);
+            if (PIPE) {
+                commit(bn, k + 1, xn, qn, en);
+                // past the chunk: an out-of-ring plane, which loads nothing
+                fetch(k + 2 <= k1 ? k + 2 : -2);
+            } else stage(bn, k + 1, xn, qn, en);
             __syncthreads();
-            hplane(bk, bn, k);
+            hplane(bk, bn, az_k);
             if (GD) pplane(bn, bk, k + 1, (k + 1) & 1);
             __syncthreads();
             if (own) {
@@ -365,7 +449,7 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
                     Hxm[c] = Hs[c][ty + 1][tx];
                     Hym[c] = Hs[c][ty][tx + 1];
                 }
-                const c64 ez = g.sdlinv[PRIM][2][k];
+                const c64 ez = ez_k;
                 c64 cc[3];
                 cc[0] = eyo * (H0[2] - Hym[2]) - ez * (H0[1] - hpy);
                 cc[1] = ez * (H0[0] - hpx) - exo * (H0[2] - Hxm[2]);
@@ -377,10 +461,9 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
                     c64 p0 = Ps[pc][ty][tx];
                     cc[0] = cc[0] - ec[0] * (gdx * (Ps[pc][ty][tx + 1] - p0));
                     cc[1] = cc[1] - ec[1] * (gdy * (Ps[pc][ty + 1][tx] - p0));
-                    cc[2] = cc[2] - ec[2] * (g.sdlinv[DUAL][2][k] * (Ps[pn][ty][tx] - p0));
+                    cc[2] = cc[2] - ec[2] * (az_k * (Ps[pn][ty][tx] - p0));
                 }
                 int64_t p = i + (int64_t)g.N[0] * j + NxNy * k;
-                c64 tzm = axis_term(g, 2, k);
                 c64 term[3] = {czero<double>(), czero<double>(), czero<double>()};
                 if (OFFD) {
                     // ∆l weights of the corner averages along each axis (reloaded
@@ -436,12 +519,15 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
                         out = cc[u] - om2 * (ec[u] * xc[u]);
                         if (OFFD) out = out - om2 * term[u];
                     } else {
-                        c64 lf = qc[u] * (Txy[u] * g.ssq[u == 2 ? DUAL : PRIM][2][k]);
+                        // q = qs·T and the output factor lf = q·T
+                        const c64 Tu = Txy[u] * (u == 2 ? szD : szP);
+                        const c64 qu = qc[u] * Tu;
+                        c64 lf = qu * Tu;
                         if (GD) {
-                            out = lf * cc[u] - om2 * ((ec[u] * (qc[u] * qc[u])) * xc[u]);
+                            out = lf * cc[u] - om2 * ((ec[u] * (qu * qu)) * xc[u]);
                         } else {
                             c64 cd = u == 0 ? tmy + tzm : (u == 1 ? tzm + tmx : tmx + tmy);
-                            out = lf * cc[u] + (cone<double>() - cd * (qc[u] * qc[u])) * xc[u];
+                            out = lf * cc[u] + (cone<double>() - cd * (qu * qu)) * xc[u];
                         }
                         if (OFFD) out = out - om2 * (lf * term[u]);
                     }

@@ -8,11 +8,12 @@ from ._lib import (MfdArgumentError, MfdError, MfdNoConvergence, MfdUnsupportedE
                    lib)
 from .core import (DUAL, EE, HH, PRIM, Ball, Box, Cuboid, Grid, MaxwellOperator, Model, ModelFull, PlaneSrc, PointSrc,
                    Sphere, Cylinder, A_csc, add_obj, add_srce, add_srcm, calc_matparams, clear_objs, clear_srcs,
-                   create_A, create_b, create_curls, create_eikL, create_field_array, create_linsys,
+                   create_A, create_b, create_curls, create_Mcs, Mcs_csc, create_eikL, create_field_array, create_linsys,
                    create_param_array, create_paramops, create_srcs, create_stretched_dls, curl_csc,
                    distweights, field_arr2vec, nX, nY, nZ, paramop_csc, set_boundft, set_kbloch, set_Npml,
                    set_omegapml, set_ωpml, solve_batch, to_scipy)
 
 from . import twod
+from . import oned
 
 __all__ = [n for n in dir() if not n.startswith("_")]

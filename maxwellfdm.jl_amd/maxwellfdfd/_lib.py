@@ -80,6 +80,8 @@ SIGNATURES = {
                                    c64, _P]),
     "mfd_curl_nnz": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int, _I64P]),
     "mfd_curl_csc": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int, _P, _P, _P]),
+    "mfd_Mcs_nnz": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int, _I64P]),
+    "mfd_Mcs_csc": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int, _P, _P, _P]),
     "mfd_matparams": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int64, _P, _P, _P, _P, _P]),
     "mfd_paramop_nnz": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int, _P, _I64P]),
     "mfd_paramop_csc": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int, _P, _P, _P, _P]),

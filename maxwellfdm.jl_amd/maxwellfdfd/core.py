@@ -334,6 +334,12 @@ def curl_csc(mdl, which):
     return _csc_call(lib().mfd_curl_nnz, lib().mfd_curl_csc, (context(), C.byref(g), which), 3 * mdl.grid.M)
 
 
+def Mcs_csc(mdl, which):
+    """Raw Julia-form CSC of the corner-mean operator Mcₑ (0) or Mcₘ (1)."""
+    g = mdl._desc()
+    return _csc_call(lib().mfd_Mcs_nnz, lib().mfd_Mcs_csc, (context(), C.byref(g), which), 3 * mdl.grid.M)
+
+
 def paramop_csc(mdl, which, parr):
     g = mdl._desc()
     parr = L.carr(parr, "F")
@@ -391,6 +397,12 @@ def create_paramops(mdl):
 def create_curls(mdl):
     """create_curls (model.jl:160-175): (Cₑ, Cₘ)."""
     return to_scipy(curl_csc(mdl, 0)), to_scipy(curl_csc(mdl, 1))
+
+
+def create_Mcs(mdl):
+    """create_Mcs (model.jl:287-306): (Mcₑ, Mcₘ), the operators that interpolate
+    the solution fields E and H to the voxel corners."""
+    return to_scipy(Mcs_csc(mdl, 0)), to_scipy(Mcs_csc(mdl, 1))
 
 
 def create_A(ft, ω, mdl, εarr=None, μarr=None):

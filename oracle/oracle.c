@@ -27,6 +27,11 @@
  *                      and brute-force differences on tiny grids (tests).
  *   param operator     model.jl:141-158 -> MaxwellBase create_paramop (absent).
  *                      Pinned by the diagonal case and S-symmetry (tests).
+ *   corner mean        model.jl:287-306 -> MaxwellBase create_mean (absent).
+ *                      Pinned by the uniform-grid weights (1/2, 1/2), the
+ *                      partition of unity on nonuniform real grids and the
+ *                      Bloch-wave eigenrelation (tests); parity vs MaxwellBase
+ *                      itself unpinned.
  *   create_A/create_b  model.jl:225-274 literally (SpGEMM, sparse minus that
  *                      drops numerical zeros, as Julia SparseArrays does).
  *   matparams          full.jl:16-70 -> assign_param!/smooth_param! (absent);
@@ -449,6 +454,44 @@ static void paramop_trip(const G *g, int which, const oc64 *parr, tlist *L) {
             }
         }
     }
+}
+
+/* create_Mcs (model.jl:287-306) -> MaxwellBase create_mean (absent):
+ * which 0 = Mcₑ (isfwd = boundft .!= EE, ∆l = s∆lₘ, ∆l′⁻¹ = s∆lₑ⁻¹),
+ * which 1 = Mcₘ (isfwd = boundft .!= HH, ∆l = s∆lₑ, ∆l′⁻¹ = s∆lₘ⁻¹).
+ * Row (w, corner c) averages F_w over the two field points beside c along w
+ * (c − 1 and c going backward, c and c + 1 going forward), each weighted by
+ * its s∆l over 2 s∆l of the corner plane; a Bloch wrap carries e⁻ⁱᵏᴸ (or its
+ * inverse), a non-Bloch ghost drops.  Formulated per row (corner), unlike the
+ * device's per-column enumeration. */
+int og_Mcs(const og_desc *d, int which, og_csc *out) {
+    G g; int rc = g_build(d, &g); if (rc) return rc;
+    int ft = which == 0 ? EE : HH;
+    tlist L = {0};
+    for (int64_t c = 0; c < g.M; ++c) {
+        int64_t q[3]; unc(&g, c, q);
+        for (int w = 0; w < 3; ++w) {
+            int isfwd = which == 0 ? (g.bft[w] != EE) : (g.bft[w] != HH);
+            int gc = g0type(&g, ft, w), gf = alt(gc);
+            int64_t st = w == 0 ? 1 : (w == 1 ? g.N[0] : g.N[0] * g.N[1]);
+            oc64 inv = 1.0 / g.sdl[gc][w][q[w]];
+            int64_t row = dof(&g, w, c);
+            /* the field point at the corner's own index */
+            tpush(&L, row, row, 0.5 * g.sdl[gf][w][q[w]] * inv);
+            /* the other field point: index q-1 (backward) or q+1 (forward) */
+            int64_t nb = q[w] + (isfwd ? 1 : -1);
+            if (nb >= 0 && nb < g.N[w]) {
+                tpush(&L, row, dof(&g, w, c + (isfwd ? st : -st)), 0.5 * g.sdl[gf][w][nb] * inv);
+            } else if (g.bloch[w]) {
+                int64_t wi = isfwd ? 0 : g.N[w] - 1;
+                oc64 ph = isfwd ? g.ph[w] : 1.0 / g.ph[w];
+                tpush(&L, row, dof(&g, w, c + (wi - q[w]) * st), 0.5 * g.sdl[gf][w][wi] * inv * ph);
+            }
+        }
+    }
+    csc_from_trip(3 * g.M, &L, out);
+    free(L.t); g_free(&g);
+    return 0;
 }
 
 int og_paramop(const og_desc *d, int which, const oc64 *parr, og_csc *out) {

@@ -71,6 +71,7 @@ int og_field_arr2vec(const og_desc *d, const oc64 *jarr, oc64 *vec);
 /* sparse assembly; CSC arrays are malloc'ed by the oracle and freed by og_csc_free */
 int og_curl(const og_desc *d, int which /* 0 = Ce, 1 = Cm */, og_csc *out);
 int og_paramop(const og_desc *d, int which /* 0 = eps, 1 = mu */, const oc64 *parr, og_csc *out);
+int og_Mcs(const og_desc *d, int which /* 0 = Mc_e, 1 = Mc_m */, og_csc *out);
 int og_create_A(const og_desc *d, double om_re, double om_im, const oc64 *eps_arr,
                 const oc64 *mu_arr, og_csc *out);
 int og_create_b(const og_desc *d, double om_re, double om_im, const oc64 *mu_arr,

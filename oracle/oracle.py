@@ -168,6 +168,15 @@ def curl(spec, which):
     return _take_csc(m, n)
 
 
+def Mcs(spec, which):
+    """Corner-mean operator Mcₑ (which 0) or Mcₘ (1) of create_Mcs (model.jl:287-306)."""
+    d = _desc(spec)
+    n = 3 * int(np.prod(_N(spec)))
+    m = CSC()
+    _check(lib().og_Mcs(C.byref(d), C.c_int(which), C.byref(m)), "Mcs")
+    return _take_csc(m, n)
+
+
 def paramop(spec, which, parr):
     d = _desc(spec)
     parr = np.asfortranarray(parr, dtype=np.complex128)

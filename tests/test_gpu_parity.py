@@ -105,6 +105,19 @@ def test_curl_csc(case, pkg, O):
 
 
 @pytest.mark.parametrize("case", list(CASES))
+def test_Mcs_csc(case, pkg, O):
+    """create_Mcs on the device (per-column enumeration) against the oracle's
+    per-corner restatement: identical structure, values to 1e-14."""
+    m = make(pkg, case)
+    s = spec_of(m)
+    for which in (0, 1):
+        say("Mcs_csc", case, which)
+        assert_csc_equal(pkg.Mcs_csc(m, which), O.Mcs(s, which), 1e-14, f"Mcs{which}")
+    Mce, Mcm = pkg.create_Mcs(m)
+    assert Mce.shape == Mcm.shape == (3 * m.grid.M,) * 2
+
+
+@pytest.mark.parametrize("case", list(CASES))
 @pytest.mark.parametrize("full", [False, True])
 def test_paramop_csc(case, full, pkg, O):
     m = make(pkg, case)
@@ -689,3 +702,66 @@ def test_model2d_tm_is_five_point_laplacian(pkg):
             for q in (p - 1, p + 1, p - N[0], p + N[0]):
                 assert row[0, q] == pytest.approx(-1.0, abs=1e-13)
             assert row.nnz == 5
+
+
+def model1d(pkg, N=40, npml=5, om=0.5):
+    D = pkg.oned
+    lp = nonuniform((N,), 31)[0]
+    m = D.ModelTEM(D.Grid1(lp))
+    D.set_ωpml(m, om)
+    D.set_Npml(m, (npml, npml))
+    D.add_obj(m, "bg", D.Segment(0.5 * (lp[0] + lp[-1]), lp[-1] - lp[0]), ε=1.0)
+    D.add_obj(m, "slab", D.Segment(0.5 * (lp[0] + lp[-1]) + 1.3, 6.4), ε=6.0 + 0.1j)
+    D.calc_matparams(m)
+    return m
+
+
+def test_model1d_tem_operator_and_solve(pkg, O):
+    """1D TEM (tem.jl) as the single-x-y-cell extruded 3D model: the Ex block
+    decouples exactly, the 1D operator equals the oracle's extruded operator
+    restricted to Ex, and the solve matches the direct solve of the 1D CSC."""
+    D = pkg.oned
+    om = 0.5
+    m = model1d(pkg, om=om)
+    s3 = spec_of(m.m3)
+    shapes, et, mt = shapes_of(m.m3)
+    ea, ma = O.matparams(s3, shapes, et, mt)
+    assert np.abs(m.m3.εarr - ea).max() <= 1e-12 * np.abs(ea).max()
+    A3 = O.to_scipy(O.create_A(s3, om, ea, ma)).tocsr()
+    ex = m.esel()
+    other = np.setdiff1d(np.arange(3 * m.grid.M), ex)
+    assert abs(A3[ex][:, other]).max() == 0 and abs(A3[other][:, ex]).max() == 0   # exact split
+    A1 = D.create_A(pkg.EE, om, m)
+    ref = A3[ex][:, ex]
+    assert abs(A1 - ref).max() <= 1e-13 * abs(ref).max()
+    assert A1.nnz <= 3 * m.grid.M   # tridiagonal
+    D.add_srce(m, D.PointSrc1(17.3))
+    je, jm = D.create_srcs(m)
+    b = D.create_b(pkg.EE, om, m, je, jm)
+    op = D.MaxwellOperator1D(m, om)
+    say("model1d tem")
+    e, st = op.solve(b, tol=1e-10)
+    xd = sla.spsolve(A1.tocsc(), b)
+    assert np.linalg.norm(e - xd) / np.linalg.norm(xd) <= 1e-7
+    assert np.linalg.norm(b - A1 @ e) / np.linalg.norm(b) <= 1.05e-10
+    h = op.h_from_e(e)
+    assert h.shape == (m.grid.M,)
+    op.close()
+
+
+def test_model1d_tem_is_three_point_laplacian(pkg):
+    """Uniform grid, no PML: the TEM operator is −d²/dz² − ω² ε with the
+    textbook 3-point stencil (diagonal 2 − ω²ε, two −1 neighbours)."""
+    D = pkg.oned
+    N = 12
+    m = D.ModelTEM(D.Grid1(np.arange(N + 1.0)))
+    D.add_obj(m, "bg", D.Segment(N / 2, float(N)), ε=3.5)
+    D.calc_matparams(m)
+    om = 0.8
+    A = D.create_A(pkg.EE, om, m).tocsr()
+    for k in range(1, N - 1):
+        row = A.getrow(k)
+        assert row[0, k] == pytest.approx(2 - om ** 2 * 3.5, abs=1e-13)
+        for q in (k - 1, k + 1):
+            assert row[0, q] == pytest.approx(-1.0, abs=1e-13)
+        assert row.nnz == 3

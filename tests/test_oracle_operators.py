@@ -302,3 +302,98 @@ def test_A_omega_zero_keeps_structure(O):
     Ce = O.to_scipy(O.curl(s, 0))
     Cm = O.to_scipy(O.curl(s, 1))
     assert abs(A0 - Cm @ Ce).max() <= 1e-14
+
+
+# ---------------------------------------------------------------- create_Mcs
+
+def mcs_block(Mc, w, N):
+    """the w-component block of a component-first corner-mean operator"""
+    S = comp(Mc, w, N)
+    return (S @ Mc @ S.T).tocsr()
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_Mcs_uniform_weights(which, O):
+    """Uniform real grid: every corner is the plain mean (1/2, 1/2) of the two
+    field points beside it along w; the boundary corner keeps 1/2 of its own
+    point (the ghost drops)."""
+    N = (5, 4, 6)
+    s = spec(N)
+    Mc = O.to_scipy(O.Mcs(s, which)).tocsr()
+    for r in range(Mc.shape[0]):
+        row = Mc.getrow(r)
+        assert set(np.round(row.data.real, 15)) <= {0.5} and np.all(row.data.imag == 0)
+        assert row.nnz in (1, 2)
+    # one boundary corner per line along w
+    n1 = sum(1 for r in range(Mc.shape[0]) if Mc.getrow(r).nnz == 1)
+    assert n1 == sum(int(np.prod(N)) // N[w] for w in range(3))
+
+
+@pytest.mark.parametrize("which,bft", [(0, (0, 0, 0)), (1, (1, 1, 1))])
+def test_Mcs_partition_of_unity(which, bft, O):
+    """Nonuniform real grid, fields at the cell centres along w and corners
+    at the primal planes (Mcₑ with boundft EE, Mcₘ with boundft HH): the ∆l
+    weights of each corner away from the boundary (whose ghost cells take
+    mirrored spacings) sum to 1 — s∆l of the two field cells over twice the
+    corner cell — which a swapped grid type or a wrong index breaks."""
+    N = (6, 5, 7)
+    s = spec(N, lprim=nonuniform(N, 3), bft=bft)
+    Mc = O.to_scipy(O.Mcs(s, which)).tocsr()
+    two = [r for r in range(Mc.shape[0]) if Mc.getrow(r).nnz == 2]
+    assert len(two) == sum(int(np.prod(N)) // N[w] * (N[w] - 1) for w in range(3))
+    n = int(np.prod(N))
+    q = np.array(np.unravel_index(np.arange(n), N, order="F"))
+    inner = [r for r in two if 1 <= q[r % 3][r // 3] <= N[r % 3] - 2]
+    assert len(inner) > len(two) // 2
+    sums = np.array([Mc.getrow(r).sum() for r in inner])
+    assert np.abs(sums - 1).max() <= 1e-14
+    w = np.array([Mc.getrow(r).data.real for r in inner])
+    assert np.all(w > 0.2) and np.all(w < 0.8) and not np.allclose(w, 0.5)
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_Mcs_bloch_eigenrelation(which, O):
+    """Uniform Bloch grid: a sampled Bloch wave along w, F_i = e^{-i k i}
+    (F(N) = e^{-ikL} F(0), the phase convention of the curls), maps to itself
+    times (1 + e^{+ik})/2 at every corner, the wrapped one included."""
+    N = (7, 5, 6)
+    kb = (0.37, -0.52, 0.81)
+    s = spec(N, bloch=(True,) * 3, kb=kb)
+    Mc = O.to_scipy(O.Mcs(s, which)).tocsr()
+    n = int(np.prod(N))
+    q = np.array(np.unravel_index(np.arange(n), N, order="F"))
+    for w in range(3):
+        F = np.zeros(3 * n, dtype=complex)
+        F[w::3] = np.exp(-1j * kb[w] * q[w])
+        G = Mc @ F
+        bft_fwd = False   # boundft EE: Mcₑ averages backward, Mcₘ forward
+        fwd = (which == 1) != bft_fwd
+        lam = 0.5 * (1 + np.exp(-1j * kb[w])) if fwd else 0.5 * (1 + np.exp(1j * kb[w]))
+        assert np.abs(G - lam * F).max() <= 1e-14
+
+
+@pytest.mark.parametrize("which,bft", [(0, (0, 0, 0)), (0, (1, 0, 1)), (1, (0, 0, 0)), (1, (0, 1, 1))])
+def test_Mcs_is_inner_average_of_paramop(which, bft, O):
+    """Pε's (Pμ's) off-diagonal block (v, w) is the corner mean of F_w (the w
+    block of Mcₑ, Mcₘ), scaled by the tensor entry at the corners, then
+    averaged half-half back to F_v along the transposed pattern of the v block
+    — on a nonuniform PML grid, with either boundary field type."""
+    N = (5, 4, 6)
+    s = spec(N, npml=((1, 0, 1), (1, 1, 0)), om=0.8, lprim=nonuniform(N, 5), bft=bft)
+    n = int(np.prod(N))
+    Mc = O.to_scipy(O.Mcs(s, which)).tocsr()
+    rng = np.random.default_rng(9)
+    for (v, w) in ((0, 1), (2, 0), (1, 2)):
+        eps = np.zeros(N + (3, 3), dtype=complex, order="F")
+        for c in range(3):
+            eps[..., c, c] = 1.0
+        x = rng.normal(size=N) + 0.2j * rng.normal(size=N)
+        eps[..., v, w] = x
+        eps[..., w, v] = x
+        P = O.to_scipy(O.paramop(s, which, eps)).tocsr()
+        Pvw = comp(P, v, N) @ P @ comp(P, w, N).T
+        Mw = mcs_block(Mc, w, N)
+        Hv = mcs_block(Mc, v, N).T.copy()
+        Hv.data[:] = 0.5
+        ref = Hv @ sp.diags(x.reshape(-1, order="F")) @ Mw
+        assert abs(Pvw - ref).max() <= 1e-14 * abs(ref).max()
