@@ -167,8 +167,8 @@ int mfd_op_apply_dev(mfd_op *op, const void *x, void *y, int64_t nrhs, int prec)
  * (W = 1/ε̄² at the primal nodes) with right-hand side r + ω⁻² ε G W Div r.
  * Div Cₘ = 0 makes every solution of A e = b a solution of the augmented
  * system, so the discretisation and the FP64 residual checked are unchanged;
- * the longitudinal near-null space of Cₘ Cₑ is moved away from zero.  On by
- * default where eligible (diagonal ε, μ = 1, ω ≠ 0).  No reference
+ * the longitudinal near-null space of Cₘ Cₑ is moved away from zero.  Off by
+ * default (opt-in; eligible with diagonal ε, μ = 1, ω ≠ 0).  No reference
  * counterpart: the reference has no solver (model.jl stops at create_A). */
 int mfd_op_set_graddiv(mfd_op *op, int enable);
 /* Tile-shape autotuner for the stencil (rows per block TY, z-planes per block
@@ -176,8 +176,8 @@ int mfd_op_set_graddiv(mfd_op *op, int enable);
  * order): times each candidate for nrhs right-hand sides and keeps the fastest
  * per precision.  Outputs {TY, KZ, order} may be null.  No reference counterpart. */
 int mfd_op_autotune(mfd_op *op, int64_t nrhs, int iters, int tile64[3], int tile32[3], double ms[2]);
-/* Fix the tile shape by hand: TY in {1,2,4} (0 = cache-reliant direct kernel),
- * KZ >= 1, xcd_order 0 or 1. */
+/* Fix the tile shape by hand: TY in {1,2,4,8} (0 = cache-reliant direct
+ * kernel), KZ >= 1, xcd_order 0 or 1. */
 int mfd_op_set_tile(mfd_op *op, int prec, int ty, int kz, int xcd_order);
 /* Average device time (ms) of one apply of the solver's stencil kernel
  * (MFD_FP64: A; MFD_MIXED: the complex-FP32 inner operator) */
