@@ -433,6 +433,26 @@ def test_solve_full_eps_tiled(case, precision, pkg, O):
     op.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_streamed(precision, pkg, O):
+    """A 100³ problem (3 M unknowns: its step exceeds the 256 MB Infinity
+    Cache, so the vector updates stream non-temporally) converges, and its
+    field meets the tolerance against the oracle's assembled A."""
+    from maxwellfdfd import workloads as W
+    m, om = W.random_boxes(N=100, npml=10, nbox=20, lam=16.0, seed=3, ndip=1)
+    pkg.calc_matparams(m)
+    je, jm = pkg.create_srcs(m)
+    b = pkg.create_b(pkg.EE, om, m, je, jm)
+    op = pkg.MaxwellOperator(m, om)
+    say("streamed solve", precision)
+    e, st = op.solve(b, tol=1e-6, precision=precision)
+    say("  iterations", st.iterations, "fp32", st.iterations_fp32, "relres", st.relres)
+    op.close()
+    A = O.to_scipy(O.create_A(spec_of(m), om, m.εarr, m.μarr))
+    rel = np.linalg.norm(b - A @ e) / np.linalg.norm(b)
+    assert st.converged and rel <= 1.05e-6
+
+
 def test_solve_bicgstab_symmetric(pkg, O):
     """BiCGSTAB chosen explicitly on a symmetric operator, against the direct solve."""
     m, om, b = c1_like(pkg)
