@@ -26,7 +26,7 @@ METRIC = "solve time to rel. residual 1e-8 + curl-curl apply GB/s; mixed vs FP64
 KERNELS = ("stencil", "upd_r", "upd_dzq")
 KERNEL_NAMES = {  # device symbols as rocprofv3 lists them (prefix match)
     "stencil": "k_apply_tiled<{R}, 1, 2",
-    "upd_r": "k_upd_r<{R},",
+    "upd_r": "k_upd_r_dot<{R},",
     "upd_dzq": "k_upd_dzq<{R},",
 }
 

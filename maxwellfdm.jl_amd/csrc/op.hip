@@ -394,16 +394,60 @@ __device__ inline void vst(T *p, T a) {
     else *p = a;
 }
 
-// r -= α t; partial sums of ||r||² and rᵀr (k_fin_r finishes the SQMR step)
+// The SQMR step's r update with the stencil's finish folded in: every block
+// sums the stencil's qᵀPq partials in the fixed order k_fin_dot uses (so σ and
+// α = ρ/σ are bitwise those of the separate finish), block 0 records them in
+// the solver scalars, and r -= α t follows with the partials of ||r||², rᵀr.
+// The scalars' flags are read once per block (thread 0), so a block never
+// splits on a flag that block 0 updates meanwhile.
 template <typename R, bool NT>
-__global__ __launch_bounds__(BS) void k_upd_r(cx<R> *__restrict__ rall, const cx<R> *__restrict__ tall, int64_t n,
-                                              int64_t vs, double *part, const Scal *scal) {
+__global__ __launch_bounds__(BS) void k_upd_r_dot(cx<R> *__restrict__ rall, const cx<R> *__restrict__ tall,
+                                                  int64_t n, int64_t vs, const double *dpart, int nbd, double *part,
+                                                  Scal *scal) {
     const int rhs = blockIdx.y;
-    const Scal &S = scal[rhs];
-    if (S.done || S.conv) return;
+    Scal &S = scal[rhs];
+    __shared__ int fl[2];
+    __shared__ double bc[4];
+    if (threadIdx.x == 0) {
+        fl[0] = S.done;
+        fl[1] = S.conv;
+        bc[2] = S.rho_re;
+        bc[3] = S.rho_im;
+    }
+    __syncthreads();
+    if (fl[0]) return;
+    double tot[2];
+    sum_partials<2>(dpart + (size_t)rhs * NPART * 4, nbd, tot);
+    if (threadIdx.x == 0) {
+        bc[0] = tot[0];
+        bc[1] = tot[1];
+    }
+    __syncthreads();
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (fl[1]) {   // converged last step: this step does nothing
+        if (lead) S.done = 1;
+        return;
+    }
+    const double sr = bc[0], si = bc[1];
+    if (sr == 0.0 && si == 0.0) {   // breakdown: no step
+        if (lead) {
+            S.sig_re = S.sig_im = 0;
+            S.breakdown = 1;
+            S.conv = 1;
+            S.alpha_re = S.alpha_im = 0;
+            S.apend_re = S.apend_im = 0;
+        }
+        return;
+    }
+    const c64 al = cdiv(mk<double>(bc[2], bc[3]), mk<double>(sr, si));
+    if (lead) {
+        S.sig_re = sr;
+        S.sig_im = si;
+        S.alpha_re = S.apend_re = al.re;
+        S.alpha_im = S.apend_im = al.im;
+    }
     cx<R> *r = rall + rhs * vs;
     const cx<R> *t = tall + rhs * vs;
-    c64 al = mk<double>(S.alpha_re, S.alpha_im);
     double acc[3] = {0, 0, 0};
     for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
         c64 v = cvt<double>(vld<NT>(&r[e])) - al * cvt<double>(vld<NT>(&t[e]));
@@ -415,7 +459,7 @@ __global__ __launch_bounds__(BS) void k_upd_r(cx<R> *__restrict__ rall, const cx
     block_partial<3>(acc, part + (size_t)rhs * NPART * 4, (int)blockIdx.x);
 }
 
-// after k_upd_r: ||r||, θ, c, τ, β, ρ of the SQMR step (one block per RHS)
+// after k_upd_r_dot: ||r||, θ, c, τ, β, ρ of the SQMR step (one block per RHS)
 __global__ __launch_bounds__(BS) void k_fin_r(const double *part, int nb, Scal *scal) {
     const int rhs = blockIdx.x;
     Scal &S = scal[rhs];
@@ -807,10 +851,12 @@ static void launch_finish(hipStream_t s, int nb, int64_t nrhs, double *part, Sca
     else if (EPI == EPI_RESID) k_fin_resid<<<(unsigned)nrhs, BS, 0, s>>>(part, nb, rn, mask);
 }
 
-// launch one operator apply
+// launch one operator apply; returns the number of partial sums per RHS its
+// reduction wrote (finish = false leaves them to the next kernel)
 template <typename R, int MODE, int EPI>
-static void launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int64_t nrhs, double *part,
-                         Scal *scal, double *rn, const int *mask = nullptr, hipEvent_t after_main = nullptr) {
+static int launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int64_t nrhs, double *part,
+                        Scal *scal, double *rn, const int *mask = nullptr, hipEvent_t after_main = nullptr,
+                        bool finish = true) {
     const DGrid<R> &g = op->G.view<R>();
     const OpData<R> &od = opd<R>(op);
     int64_t vs = 3 * g.M;
@@ -840,8 +886,8 @@ static void launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int
             else k_apply_tiled<R, MODE, EPI, 1, false><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
         }
         if (after_main) hipEventRecord(after_main, s);
-        launch_finish<EPI>(s, nx * ny * nch, nrhs, part, scal, rn, mask);
-        return;
+        if (finish) launch_finish<EPI>(s, nx * ny * nch, nrhs, part, scal, rn, mask);
+        return nx * ny * nch;
     }
     dim3 grid(ngrid(g.M), (unsigned)nrhs);
     if (op->full) {
@@ -852,14 +898,8 @@ static void launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int
         else k_apply<R, MODE, false, false, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, scal, mask);
     }
     if (after_main) hipEventRecord(after_main, s);
-    launch_finish<EPI>(s, (int)grid.x, nrhs, part, scal, rn, mask);
-}
-
-template <typename R>
-static void launch_upd_r(dim3 gv, hipStream_t s, cx<R> *r, const cx<R> *t, int64_t n, double *part, const Scal *scal,
-                         bool nt) {
-    if (nt) k_upd_r<R, true><<<gv, BS, 0, s>>>(r, t, n, n, part, scal);
-    else k_upd_r<R, false><<<gv, BS, 0, s>>>(r, t, n, n, part, scal);
+    if (finish) launch_finish<EPI>(s, (int)grid.x, nrhs, part, scal, rn, mask);
+    return (int)grid.x;
 }
 
 template <typename R>
@@ -879,18 +919,28 @@ static bool stream_nt(int64_t n, int64_t nrhs, size_t vbytes) {
     return foot > 256.0 * 1024 * 1024;
 }
 
+// One SQMR step: stencil t = P q with qᵀt partials, r update (with the α
+// finish folded in), finish of ||r||, rᵀr, d/z/q update.  E (optional): six
+// events around the main kernels, for timing in context.
 template <typename R>
-static void launch_iter(mfd_op *op, int64_t nrhs) {
+static void launch_iter(mfd_op *op, int64_t nrhs, hipEvent_t *E = nullptr) {
     Workspace &w = op->ws;
     int64_t n = w.n;
     hipStream_t s = op->stream;
     dim3 gv(ngrid(n), (unsigned)nrhs);
     size_t pslot = (size_t)nrhs * NPART * 4;
     const bool nt = stream_nt(n, nrhs, sizeof(cx<R>));
-    launch_apply<R, MODE_P, EPI_DOT>(op, (const cx<R> *)w.q, (cx<R> *)w.t, nullptr, nrhs, w.part, w.scal, nullptr);
-    launch_upd_r<R>(gv, s, (cx<R> *)w.r, (const cx<R> *)w.t, n, w.part + pslot, w.scal, nt);
+    if (E) hipEventRecord(E[0], s);
+    const int nbd = launch_apply<R, MODE_P, EPI_DOT>(op, (const cx<R> *)w.q, (cx<R> *)w.t, nullptr, nrhs, w.part,
+                                                     w.scal, nullptr, nullptr, E ? E[1] : nullptr, false);
+    if (E) hipEventRecord(E[2], s);
+    if (nt) k_upd_r_dot<R, true><<<gv, BS, 0, s>>>((cx<R> *)w.r, (const cx<R> *)w.t, n, n, w.part, nbd, w.part + pslot, w.scal);
+    else k_upd_r_dot<R, false><<<gv, BS, 0, s>>>((cx<R> *)w.r, (const cx<R> *)w.t, n, n, w.part, nbd, w.part + pslot, w.scal);
+    if (E) hipEventRecord(E[3], s);
     k_fin_r<<<(unsigned)nrhs, BS, 0, s>>>(w.part + pslot, (int)gv.x, w.scal);
+    if (E) hipEventRecord(E[4], s);
     launch_upd_dzq<R>(gv, s, (cx<R> *)w.d, (cx<R> *)w.z, (cx<R> *)w.q, (const cx<R> *)w.r, n, w.scal, nt);
+    if (E) hipEventRecord(E[5], s);
 }
 
 }  // namespace mfd
@@ -1120,19 +1170,8 @@ int op_bench_iteration_impl(mfd_op *op, int64_t nrhs, int prec, int iters, doubl
     for (auto &e : ev)
         if (hipEventCreate(&e) != hipSuccess) { rc = MFD_ERR_HIP; break; }
     auto iteration = [&](hipEvent_t *E) {
-        if (E) hipEventRecord(E[0], s);
-        if (prec == MFD_FP64)
-            launch_apply<double, MODE_P, EPI_DOT>(op, (const c64 *)w.q, (c64 *)w.t, nullptr, nrhs, w.part, w.scal, nullptr, nullptr, E ? E[1] : nullptr);
-        else launch_apply<float, MODE_P, EPI_DOT>(op, (const c32 *)w.q, (c32 *)w.t, nullptr, nrhs, w.part, w.scal, nullptr, nullptr, E ? E[1] : nullptr);
-        if (E) hipEventRecord(E[2], s);
-        if (prec == MFD_FP64) launch_upd_r<double>(gv, s, (c64 *)w.r, (const c64 *)w.t, n, w.part + ps, w.scal, stream_nt(n, nrhs, sizeof(c64)));
-        else launch_upd_r<float>(gv, s, (c32 *)w.r, (const c32 *)w.t, n, w.part + ps, w.scal, stream_nt(n, nrhs, sizeof(c32)));
-        if (E) hipEventRecord(E[3], s);
-        k_fin_r<<<(unsigned)nrhs, BS, 0, s>>>(w.part + ps, (int)gv.x, w.scal);
-        if (E) hipEventRecord(E[4], s);
-        if (prec == MFD_FP64) launch_upd_dzq<double>(gv, s, (c64 *)w.d, (c64 *)w.z, (c64 *)w.q, (const c64 *)w.r, n, w.scal, stream_nt(n, nrhs, sizeof(c64)));
-        else launch_upd_dzq<float>(gv, s, (c32 *)w.d, (c32 *)w.z, (c32 *)w.q, (const c32 *)w.r, n, w.scal, stream_nt(n, nrhs, sizeof(c32)));
-        if (E) hipEventRecord(E[5], s);
+        if (prec == MFD_FP64) launch_iter<double>(op, nrhs, E);
+        else launch_iter<float>(op, nrhs, E);
     };
     if (!rc) {
         k_bench_scal<<<(unsigned)((nrhs + 63) / 64), 64, 0, s>>>(w.scal, nrhs);
