@@ -1136,8 +1136,8 @@ int op_bench_apply_impl(mfd_op *op, int64_t nrhs, int prec, int iters, double *m
     return time_apply(op, nrhs, prec, prec == MFD_FP64 ? MODE_A : MODE_P, iters, ms);
 }
 
-// Scalars of an active iteration with α = β = 0, so that repeated launches of
-// one solver kernel leave the vectors bounded (timing only)
+// Scalars of an active iteration that never converges (ρ = 1, τ = 1, no
+// iteration cap), for timing complete solver iterations back to back
 __global__ void k_bench_scal(Scal *scal, int64_t nrhs) {
     int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (r >= nrhs) return;
@@ -1602,10 +1602,13 @@ void solve_opts_default(mfd_solve_opts *o) {
     o->use_graph = 1;
 }
 
+// stalled (optional): set when an FP32 cycle ended because its probes stopped
+// improving, rather than at its target or budget
 template <typename R>
 static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double rel_target, const int *active_d,
                      long long budget, long long *iters_out, const void *b = nullptr, const void *e = nullptr,
-                     const double *bn = nullptr) {
+                     const double *bn = nullptr, bool *stalled = nullptr) {
+    if (stalled) *stalled = false;
     Workspace &w = op->ws;
     hipStream_t s = op->stream;
     int64_t n = w.n;
@@ -1667,7 +1670,10 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
             for (int64_t r = 0; r < nrhs; ++r) worst = std::max(worst, bn[r] > 0 ? sqrt(w.rn_h[r] / bn[r]) : 0.0);
             if (getenv("MFD_SOLVE_TRACE")) fprintf(stderr, "[mfd probe] it %lld true residual %.3e\n", mx, worst);
             if (worst > 0.5 * best_probe) {
-                if (++stall >= stall_max) break;
+                if (++stall >= stall_max) {
+                    if (stalled) *stalled = true;
+                    break;
+                }
             } else stall = 0;
             best_probe = std::min(best_probe, worst);
         }
@@ -1974,9 +1980,12 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
     long long total_it = 0, cycles = 0, it32 = 0;
     double worst = 0, prev_worst = 0;
     bool conv = false;
-    // FP32 cycles run while each gains at least 10x on the FP64 residual; a
-    // cycle that gains less switches the rest of the solve to FP64 cycles
+    // FP32 cycles run while each gains at least 10x on the FP64 residual and
+    // ends at its target; a cycle that gains less, or whose probes stopped
+    // improving (the FP32 recurrence stagnated; a restart from the refined
+    // residual stagnates again), switches the rest of the solve to FP64 cycles
     int cyc_prec = method == MFD_BICGSTAB ? MFD_FP64 : prec;
+    bool stalled32 = false;
     while (true) {
         launch_apply<double, MODE_A, EPI_RESID>(op, (const c64 *)e, (c64 *)w.r64, (const c64 *)b, nrhs,
                                                 w.part + 2 * pslot, nullptr, w.rn);
@@ -1993,7 +2002,7 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
         }
         if (all) { conv = true; break; }
         if (total_it >= o.maxit) break;
-        if (cyc_prec == MFD_MIXED && cycles > 0 && worst > 0.1 * prev_worst) cyc_prec = MFD_FP64;
+        if (cyc_prec == MFD_MIXED && cycles > 0 && (worst > 0.1 * prev_worst || stalled32)) cyc_prec = MFD_FP64;
         prev_worst = worst;
         MFD_HIP(hipMemcpyAsync(active_d, active.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s));
         if (op->gd) {
@@ -2012,7 +2021,7 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
         int rc;
         if (method == MFD_BICGSTAB) rc = run_inner_bicgstab(op, nrhs, &o, target, active_d, o.maxit - total_it, &it);
         else if (cyc_prec == MFD_MIXED)
-            rc = run_inner<float>(op, nrhs, &o, target, active_d, o.maxit - total_it, &it, b, e, bn.data());
+            rc = run_inner<float>(op, nrhs, &o, target, active_d, o.maxit - total_it, &it, b, e, bn.data(), &stalled32);
         else rc = run_inner<double>(op, nrhs, &o, target, active_d, o.maxit - total_it, &it);
         if (rc) return rc;
         total_it += it;
@@ -2061,7 +2070,7 @@ struct BatchProb {
     double bn = 0, worst = 1, prev = 0, best_probe = 1e300;
     int stall = 0;
     long long it_total = 0, it32 = 0, cycles = 0, next_probe = 0;
-    bool conv = false, done = false;
+    bool conv = false, done = false, stalled32 = false;   // stalled32: an FP32 cycle stagnated
 };
 }  // namespace
 
@@ -2153,7 +2162,7 @@ int solve_batch_dev_impl(int count, mfd_op *const *ops, const void *const *b, vo
                 return MFD_OK;
             }
             if (B.it_total >= o.maxit || B.cycles > 200) { B.done = true; return MFD_OK; }
-            if (B.prec == MFD_MIXED && B.cycles > 0 && rel > 0.1 * B.prev) B.prec = MFD_FP64;
+            if (B.prec == MFD_MIXED && B.cycles > 0 && (rel > 0.1 * B.prev || B.stalled32)) B.prec = MFD_FP64;
             B.prev = rel;
             if (op->gd) {
                 k_gd_psi<<<gv, BS, 0, s>>>(op->G.d64, op->d64.wnode, (const c64 *)w.r64, (c64 *)w.psi, n);
@@ -2256,8 +2265,10 @@ int solve_batch_dev_impl(int count, mfd_op *const *ops, const void *const *b, vo
                     BCHK(hipMemcpyAsync(w.rn_h, w.rn, sizeof(double), hipMemcpyDeviceToHost, s));
                     BCHK(hipStreamSynchronize(s));
                     double pr = B.bn > 0 ? sqrt(w.rn_h[0] / B.bn) : 0.0;
-                    if (pr > 0.5 * B.best_probe) end = ++B.stall >= 2;
-                    else B.stall = 0;
+                    if (pr > 0.5 * B.best_probe) {
+                        end = ++B.stall >= 2;
+                        if (end) B.stalled32 = true;
+                    } else B.stall = 0;
                     B.best_probe = std::min(B.best_probe, pr);
                 }
                 if (end) {
