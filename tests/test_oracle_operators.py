@@ -397,3 +397,24 @@ def test_Mcs_is_inner_average_of_paramop(which, bft, O):
         Hv.data[:] = 0.5
         ref = Hv @ sp.diags(x.reshape(-1, order="F")) @ Mw
         assert abs(Pvw - ref).max() <= 1e-14 * abs(ref).max()
+
+
+@pytest.mark.parametrize("bft", [(0, 0, 0), (1, 0, 1)])
+@pytest.mark.parametrize("bloch", [(False,) * 3, (True, False, True)])
+def test_A_hh_is_ee_of_flipped_model(bft, bloch, O):
+    """E↔H duality: the HH operator Cₑ (Pε \\ Cₘ) − ω² Pμ of a model equals,
+    entry for entry, the EE operator of the same grid with every boundft
+    flipped and ε, μ swapped."""
+    N = (5, 4, 6)
+    kw = dict(npml=((1, 0, 1), (1, 1, 0)), om=0.8, lprim=nonuniform(N, 5), bloch=bloch, kb=(0.2, 0.0, -0.3))
+    s = spec(N, bft=bft, **kw)
+    sf = spec(N, bft=tuple(1 - b for b in bft), **kw)
+    rng = np.random.default_rng(1)
+    eps = np.zeros(N + (3, 3), dtype=complex, order="F")
+    mu = np.zeros_like(eps)
+    for c in range(3):
+        eps[..., c, c] = 1 + 5 * rng.random(N) + 0.1j
+        mu[..., c, c] = 1 + rng.random(N)
+    Ahh = O.to_scipy(O.create_A(s, 0.7, eps, mu, ft=1))
+    Aee = O.to_scipy(O.create_A(sf, 0.7, mu, eps, ft=0))
+    assert (Ahh != Aee).nnz == 0 and Ahh.nnz == Aee.nnz
