@@ -154,8 +154,13 @@ int mfd_A_csc(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_
 int mfd_create_b(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr,
                  const mfd_c64 *mu_arr, const mfd_c64 *je, const mfd_c64 *jm, mfd_c64 *b);
 
-/* Matrix-free operator for the A of create_A (model.jl:235-237), device
- * resident.  eps_arr/mu_arr as produced by create_paramops (host). */
+/* Matrix-free operator for the A of create_A (model.jl:225-246), device
+ * resident.  eps_arr/mu_arr as produced by create_paramops (host).
+ * ft = EE: A = Cₘ (Pμ \ Cₑ) − ω² Pε; ft = HH: A = Cₑ (Pε \ Cₘ) − ω² Pμ, built as
+ * the identical EE operator of the same grid with every boundft flipped and
+ * ε, μ swapped.  Any boundft; the tiled stencils serve boundft = EE on every
+ * axis with μ = 1 (EE) — other operators use the direct kernel.  Off-diagonal
+ * ε (EE) needs boundft = EE on every axis, off-diagonal μ (HH) boundft = HH. */
 int mfd_op_create(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr,
                   const mfd_c64 *mu_arr, mfd_op **out);
 int mfd_op_destroy(mfd_op *op);
@@ -207,7 +212,8 @@ int mfd_solve_batch(int count, mfd_op *const *ops, const mfd_c64 *const *b, mfd_
 int mfd_solve_batch_dev(int count, mfd_op *const *ops, const void *const *b, void *const *e,
                         const mfd_solve_opts *o, mfd_solve_stats *st);
 /* h_from_e (model.jl:276-279): h = (i/ω) Pμ⁻¹ (Cₑ e + jₘ) with the operator's
- * grid, μ and ω; e, jm, h host DOF vectors of 3M values (jm may be null). */
+ * grid, μ and ω; e, jm, h host DOF vectors of 3M values (jm may be null).
+ * Needs an EE operator with boundft = EE on every axis (as e_from_h does). */
 int mfd_h_from_e(mfd_op *op, const mfd_c64 *e, const mfd_c64 *jm, mfd_c64 *h);
 /* e_from_h (model.jl:281-284): e = (−i/ω) Pε⁻¹ (Cₘ h − jₑ); diagonal ε only
  * (MFD_ERR_UNSUPPORTED for an operator with off-diagonal ε); je may be null. */
@@ -215,7 +221,8 @@ int mfd_e_from_h(mfd_op *op, const mfd_c64 *h, const mfd_c64 *je, mfd_c64 *e);
 int mfd_solve_adjoint(mfd_op *op, const mfd_c64 *g, mfd_c64 *lam, const mfd_solve_opts *o,
                       mfd_solve_stats *st);
 /* ∂f/∂ε per E DOF for real f with Aᵀλ = (∂f/∂e)ᵀ: grad = 2ω² Re(λ ∘ e)
- * (derivative through the diagonal of −ω²Pε). */
+ * (derivative through the diagonal of −ω²Pε; for an HH operator, ∂f/∂μ per
+ * H DOF through −ω²Pμ). */
 int mfd_adjoint_grad(mfd_op *op, const mfd_c64 *e, const mfd_c64 *lam, double *grad);
 
 #ifdef __cplusplus

@@ -47,13 +47,14 @@ __device__ inline void dof2cp(const DGrid<R> &g, int64_t e, int &c, int64_t &p) 
     if (g.cmpfirst) { c = (int)(e % 3); p = e / 3; }
     else { c = (int)(e / g.M); p = e - (int64_t)c * g.M; }
 }
-// T (or T⁻¹) for E_c at (i,j,k): product over axes of sqrt(s∆l) at E_c's grid type
+// T (or T⁻¹) for E_c at (i,j,k): product over axes of sqrt(s∆l) at E_c's grid
+// type (which each axis's boundft sets)
 template <typename R>
 __device__ inline cx<R> tfac(const DGrid<R> &g, int c, int i, int j, int k, bool inv) {
     const cx<R> *const(*t)[3] = inv ? g.ssqinv : g.ssq;
-    cx<R> a = t[c == 0 ? DUAL : PRIM][0][i];
-    cx<R> b = t[c == 1 ? DUAL : PRIM][1][j];
-    cx<R> d = t[c == 2 ? DUAL : PRIM][2][k];
+    cx<R> a = t[gtype(g.bft, EE, c, 0)][0][i];
+    cx<R> b = t[gtype(g.bft, EE, c, 1)][1][j];
+    cx<R> d = t[gtype(g.bft, EE, c, 2)][2][k];
     return (a * b) * d;
 }
 
@@ -81,8 +82,10 @@ __device__ inline cx<R> ldX(const DGrid<R> &g, const OpData<R> &od, const cx<R> 
     return wr ? v * ph : v;
 }
 
-// H_u at (i,j,k) = ∂_v X_w − ∂_w X_v (forward, v = u+1, w = u+2), times 1/μ_u;
-// one index may be -1 (backward ghost of Cₘ)
+// H_u at (i,j,k) = ∂_v X_w − ∂_w X_v (v = u+1, w = u+2), times 1/μ_u.  Cₑ
+// differences run forward along an axis whose boundft is EE and backward where
+// it is HH; Cₘ the other way (so one index may be -1 or N: the outer curl's
+// ghost, wrapped with the Bloch phase or zero).
 template <typename R, int MODE, bool MU>
 __device__ inline cx<R> Hval(const DGrid<R> &g, const OpData<R> &od, const cx<R> *__restrict__ x, int u, int i,
                              int j, int k) {
@@ -90,20 +93,27 @@ __device__ inline cx<R> Hval(const DGrid<R> &g, const OpData<R> &od, const cx<R>
     bool wr = false;
     int id[3] = {i, j, k};
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < 3; ++a) {
         if (id[a] < 0) {
             if (!g.bloch[a]) return czero<R>();
             id[a] += g.N[a]; ph = ph * g.phinv[a]; wr = true;
+        } else if (id[a] >= g.N[a]) {
+            if (!g.bloch[a]) return czero<R>();
+            id[a] -= g.N[a]; ph = ph * g.ph[a]; wr = true;
         }
+    }
     int v = u == 0 ? 1 : (u == 1 ? 2 : 0), w = u == 0 ? 2 : (u == 1 ? 0 : 1);
-    int iv[3] = {id[0], id[1], id[2]}, iw[3] = {id[0], id[1], id[2]};
-    iv[v] += 1;
-    iw[w] += 1;
-    cx<R> x0w = ldX<R, MODE>(g, od, x, w, id[0], id[1], id[2]);
-    cx<R> x1w = ldX<R, MODE>(g, od, x, w, iv[0], iv[1], iv[2]);
-    cx<R> x0v = ldX<R, MODE>(g, od, x, v, id[0], id[1], id[2]);
-    cx<R> x1v = ldX<R, MODE>(g, od, x, v, iw[0], iw[1], iw[2]);
-    cx<R> h = g.sdlinv[DUAL][v][id[v]] * (x1w - x0w) - g.sdlinv[DUAL][w][id[w]] * (x1v - x0v);
+    // ∂_a X_c at id without the ∆l factor, in Cₑ's direction along a
+    auto dif = [&](int a, int c) -> cx<R> {
+        int i1[3] = {id[0], id[1], id[2]};
+        if (g.bft[a] == EE) {
+            i1[a] += 1;
+            return ldX<R, MODE>(g, od, x, c, i1[0], i1[1], i1[2]) - ldX<R, MODE>(g, od, x, c, id[0], id[1], id[2]);
+        }
+        i1[a] -= 1;
+        return ldX<R, MODE>(g, od, x, c, id[0], id[1], id[2]) - ldX<R, MODE>(g, od, x, c, i1[0], i1[1], i1[2]);
+    };
+    cx<R> h = g.sdlinv[g0type(g.bft, HH, v)][v][id[v]] * dif(v, w) - g.sdlinv[g0type(g.bft, HH, w)][w][id[w]] * dif(w, v);
     if (MU) {
         int64_t p = id[0] + (int64_t)g.N[0] * (id[1] + (int64_t)g.N[1] * id[2]);
         h = h * od.muinv[3 * p + u];
@@ -111,7 +121,9 @@ __device__ inline cx<R> Hval(const DGrid<R> &g, const OpData<R> &od, const cx<R>
     return wr ? h * ph : h;
 }
 
-// curl-curl diagonal for E_u at (i,j,k): Σ over the two derivative axes
+// curl-curl diagonal for E_u at (i,j,k): Σ over the two derivative axes (Cₘ
+// backward along an EE axis, taking H at p and p−1; forward along an HH axis,
+// taking H at p and p+1; a non-Bloch ghost H is zero)
 template <typename R, bool MU>
 __device__ inline cx<R> ccdiag(const DGrid<R> &g, const OpData<R> &od, int u, int i, int j, int k) {
     int id[3] = {i, j, k};
@@ -122,23 +134,24 @@ __device__ inline cx<R> ccdiag(const DGrid<R> &g, const OpData<R> &od, int u, in
         int a = t == 0 ? (u == 0 ? 1 : (u == 1 ? 2 : 0)) : (u == 0 ? 2 : (u == 1 ? 0 : 1));
         int hc = t == 0 ? (u == 0 ? 2 : (u == 1 ? 0 : 1)) : (u == 0 ? 1 : (u == 1 ? 2 : 0));  // H component
         int ia = id[a], Na = g.N[a];
+        const bool fwd = g.bft[a] == HH;   // Cₘ direction along a
+        const int gh = g0type(g.bft, HH, a);
+        int io = fwd ? (ia < Na - 1 ? ia + 1 : 0) : (ia > 0 ? ia - 1 : Na - 1);   // the other H index
         cx<R> m0 = cone<R>(), m1 = cone<R>();
         int64_t sta = a == 0 ? 1 : (a == 1 ? (int64_t)g.N[0] : (int64_t)g.N[0] * g.N[1]);
         if (MU) {
             m0 = od.muinv[3 * p + hc];
-            int64_t pm = ia > 0 ? p - sta : p + (int64_t)(Na - 1) * sta;
-            m1 = od.muinv[3 * pm + hc];
+            m1 = od.muinv[3 * (p + (int64_t)(io - ia) * sta) + hc];
         }
-        cx<R> ie = g.sdlinv[PRIM][a][ia];
+        cx<R> ie = g.sdlinv[g0type(g.bft, EE, a)][a][ia];
         cx<R> term;
         if (Na == 1 && g.bloch[a]) {
             cx<R> f = mk<R>((R)2, (R)0) - g.ph[a] - g.phinv[a];
-            term = (ie * (g.sdlinv[DUAL][a][0] * m0)) * f;
-        } else if (ia == 0 && !g.bloch[a]) {
-            term = ie * (g.sdlinv[DUAL][a][0] * m0);
+            term = (ie * (g.sdlinv[gh][a][0] * m0)) * f;
+        } else if (!g.bloch[a] && (fwd ? ia == Na - 1 : ia == 0)) {
+            term = ie * (g.sdlinv[gh][a][ia] * m0);
         } else {
-            int im = ia > 0 ? ia - 1 : Na - 1;
-            term = ie * (g.sdlinv[DUAL][a][ia] * m0 + g.sdlinv[DUAL][a][im] * m1);
+            term = ie * (g.sdlinv[gh][a][ia] * m0 + g.sdlinv[gh][a][io] * m1);
         }
         s = s + term;
     }
@@ -276,7 +289,9 @@ __global__ __launch_bounds__(BS) void k_apply(DGrid<R> g, OpData<R> od, const cx
         for (int64_t p = blockIdx.x * (int64_t)BS + threadIdx.x; p < g.M; p += (int64_t)gridDim.x * BS) {
             int i, j, k;
             cell3(g, p, i, j, k);
-            cx<R> H[3], Hm[3][3];   // H_c(p); Hm[c][a] = H_c(p - â)
+            // H_c(p) and Hm[c][a] = H_c(p ∓ â), the neighbour Cₘ's difference
+            // along a takes (p − â along an EE axis, p + â along an HH axis)
+            cx<R> H[3], Hm[3][3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) H[c] = Hval<R, MODE, MU>(g, od, x, c, i, j, k);
 #pragma unroll
@@ -285,15 +300,17 @@ __global__ __launch_bounds__(BS) void k_apply(DGrid<R> g, OpData<R> od, const cx
                 for (int a = 0; a < 3; ++a) {
                     if (a == c) continue;
                     int id[3] = {i, j, k};
-                    id[a] -= 1;
+                    id[a] += g.bft[a] == EE ? -1 : 1;
                     Hm[c][a] = Hval<R, MODE, MU>(g, od, x, c, id[0], id[1], id[2]);
                 }
             int id[3] = {i, j, k};
 #pragma unroll
             for (int u = 0; u < 3; ++u) {
                 int v = u == 0 ? 1 : (u == 1 ? 2 : 0), w = u == 0 ? 2 : (u == 1 ? 0 : 1);
-                // E_u = ∂_v H_w − ∂_w H_v (backward)
-                cx<R> cc = g.sdlinv[PRIM][v][id[v]] * (H[w] - Hm[w][v]) - g.sdlinv[PRIM][w][id[w]] * (H[v] - Hm[v][w]);
+                // E_u = ∂_v H_w − ∂_w H_v (backward along EE axes, forward along HH axes)
+                cx<R> dv = g.bft[v] == EE ? H[w] - Hm[w][v] : Hm[w][v] - H[w];
+                cx<R> dw = g.bft[w] == EE ? H[v] - Hm[v][w] : Hm[v][w] - H[v];
+                cx<R> cc = g.sdlinv[g0type(g.bft, EE, v)][v][id[v]] * dv - g.sdlinv[g0type(g.bft, EE, w)][w][id[w]] * dw;
                 int64_t e = dofidx(g, u, p);
                 cx<R> xu = x[e];
                 cx<R> out;
@@ -865,7 +882,7 @@ static int launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int6
     int kz = std::is_same<R, double>::value ? op->tile64[1] : op->tile32[1];
     int xr = std::is_same<R, double>::value ? op->tile64[2] : op->tile32[2];
     if (MODE == MODE_P && op->gd && ty == 8) ty = 4;   // the grad-div tile keeps two more LDS planes
-    if (!op->has_mu && ty > 0) {
+    if (!op->has_mu && op->ee_all && ty > 0) {
         int nx = (g.N[0] + 63) / 64, ny = (g.N[1] + ty - 1) / ty;
         int nch = (g.N[2] + kz - 1) / kz;
         while ((int64_t)nx * ny * nch > NPART) { kz *= 2; nch = (g.N[2] + kz - 1) / kz; }
@@ -951,12 +968,22 @@ using namespace mfd;
 
 int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr,
                    const mfd_c64 *mu_arr, mfd_op **out) {
-    if (ft != MFD_EE) { set_error("matrix-free operator: only ft = EE is implemented"); return MFD_ERR_UNSUPPORTED; }
-    for (int a = 0; a < 3; ++a)
-        if (g->boundft[a] != MFD_EE) { set_error("matrix-free operator: boundft must be EE on every axis"); return MFD_ERR_UNSUPPORTED; }
+    if (ft != MFD_EE && ft != MFD_HH) { set_error("ft must be EE or HH"); return MFD_ERR_ARG; }
+    // E↔H duality: the HH operator Cₑ (Pε \ Cₘ) − ω² Pμ of a model is, entry
+    // for entry, the EE operator of the same grid with every boundft flipped
+    // and ε, μ swapped, so an HH operator is built as that EE operator
+    mfd_grid gd = *g;
+    if (ft == MFD_HH) {
+        for (int a = 0; a < 3; ++a) gd.boundft[a] = gd.boundft[a] == MFD_EE ? MFD_HH : MFD_EE;
+        std::swap(eps_arr, mu_arr);
+    }
+    g = &gd;
     mfd_op *op = new mfd_op();
     op->ctx = ctx;
     op->stream = ctx->stream;
+    op->ft = ft;
+    for (int a = 0; a < 3; ++a)
+        if (g->boundft[a] != MFD_EE) op->ee_all = 0;
     int rc = op->G.create(g, ctx->stream);
     if (rc) { delete op; return rc; }
     hipStream_t s = ctx->stream;
@@ -968,7 +995,16 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
     auto bail = [&](int code) { delete op; return code; };
     if ((rc = upload_param(ctx, op->G, eps_arr, E, &full))) return bail(rc);
     if ((rc = upload_param(ctx, op->G, mu_arr, U, &mufull))) return bail(rc);
-    if (mufull) { set_error("create_A: Pμ must be diagonal (Pμ \\ Cₑ), model.jl:236"); return bail(MFD_ERR_UNSUPPORTED); }
+    if (mufull) {
+        set_error(ft == MFD_EE ? "create_A: Pμ must be diagonal (Pμ \\ Cₑ), model.jl:236"
+                               : "create_A: Pε must be diagonal (Pε \\ Cₘ), model.jl:239");
+        return bail(MFD_ERR_UNSUPPORTED);
+    }
+    if (full && !op->ee_all) {
+        set_error(ft == MFD_EE ? "matrix-free operator: off-diagonal ε needs boundft = EE on every axis"
+                               : "matrix-free HH operator: off-diagonal μ needs boundft = HH on every axis");
+        return bail(MFD_ERR_UNSUPPORTED);
+    }
     // is μ identically 1?
     {
         std::vector<int> dummy;
@@ -1018,7 +1054,8 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
     // (the wrap entries of Cₑ and Cₘ are conjugate, not transposed, images)
     for (int a = 0; a < 3; ++a)
         if (op->G.d64.bloch[a] && fabs(op->G.d64.ph[a].im) > 1e-14) op->sym = 0;
-    op->gd_ok = !full && !op->has_mu && !(om2.re == 0.0 && om2.im == 0.0);
+    op->gd_ok = !full && !op->has_mu && op->ee_all && !(om2.re == 0.0 && om2.im == 0.0);
+    if (!op->ee_all) op->tile64[0] = op->tile32[0] = 0;   // the direct kernel serves other boundaries
     op->gd = 0;   // opt-in: it helps small PML-dominated grids and slows larger ones
     k_wnode<<<nb(M), BS, 0, s>>>(op->G.d64, epsd, wn, wn32);
     if (op->gd) k_jacobi_gd<<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q);
@@ -1200,7 +1237,7 @@ int op_bench_iteration_impl(mfd_op *op, int64_t nrhs, int prec, int iters, doubl
 // operator P on FP64 or complex-FP32 vectors) for each candidate (TY, KZ, tile
 // order) and keeps the fastest per precision.
 int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best32, double *ms_out) {
-    if (op->has_mu) {   // the direct kernel serves these operators
+    if (op->has_mu || !op->ee_all) {   // the direct kernel serves these operators
         op->tile64[0] = op->tile32[0] = 0;
         if (best64) best64[0] = best64[1] = best64[2] = 0;
         if (best32) best32[0] = best32[1] = best32[2] = 0;
@@ -1408,7 +1445,7 @@ static void launch_cocg_iter(mfd_op *op, int64_t nrhs, int par) {
     cx<R> *po = cocg_p<R>(w, par), *pn = cocg_p<R>(w, par ^ 1);
     const DGrid<R> &g = op->G.view<R>();
     int ty = std::is_same<R, double>::value ? op->tile64[0] : op->tile32[0];
-    if (!op->full && !op->has_mu && !op->gd && ty > 0) {
+    if (!op->full && !op->has_mu && !op->gd && op->ee_all && ty > 0) {
         int kz = std::is_same<R, double>::value ? op->tile64[1] : op->tile32[1];
         int xr = std::is_same<R, double>::value ? op->tile64[2] : op->tile32[2];
         int nx = (g.N[0] + 63) / 64, ny = (g.N[1] + ty - 1) / ty, nch = (g.N[2] + kz - 1) / kz;
@@ -2455,6 +2492,10 @@ __global__ __launch_bounds__(BS) void k_e_from_h(DGrid<double> g, OpData<double>
 
 // which = 0: out = h_from_e(in, src = jₘ); which = 1: out = e_from_h(in, src = jₑ)
 int field_convert_impl(mfd_op *op, int which, const mfd_c64 *in, const mfd_c64 *src, mfd_c64 *out) {
+    if (op->ft != MFD_EE || !op->ee_all) {
+        set_error("h_from_e/e_from_h: operator of an EE model with boundft = EE on every axis");
+        return MFD_ERR_UNSUPPORTED;
+    }
     if (which == 1 && op->full) { set_error("e_from_h: only diagonal ε is supported (Pε \\ is a sparse solve otherwise)"); return MFD_ERR_UNSUPPORTED; }
     if (op->omega.re == 0.0 && op->omega.im == 0.0) { set_error("h_from_e/e_from_h need ω ≠ 0"); return MFD_ERR_ARG; }
     int64_t n = 3 * op->G.h.M;

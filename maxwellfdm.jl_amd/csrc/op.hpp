@@ -68,12 +68,14 @@ struct mfd_op {
     mfd::OpData<float> d32;
     mfd::Workspace ws;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // tiled stencil shape (TY rows per block, KZ planes per chunk); TY = 0
+    // tiled stencil shape (TY rows per block, KZ planes per chunk, and
+    // XCD-contiguous tile order (1) or plain dispatch order (0)); TY = 0
     // selects the direct kernel
-    // and XCD-contiguous tile order (1) or plain dispatch order (0)
     int tile64[3] = {4, 16, 1};
     int tile32[3] = {4, 16, 1};
     int sym = 1;              // complex-symmetric up to S_E (no complex Bloch phase)
     int gd = 0;               // grad-div augmented inner operator in use
     int gd_ok = 0;            // eligible (diagonal ε, μ = 1)
+    int ft = 0;               // formulation the caller asked for (MFD_EE, or MFD_HH served as the dual EE operator)
+    int ee_all = 1;           // boundft = EE on every axis (the tiled stencils and field conversions need it)
 };

@@ -474,7 +474,10 @@ def solve_batch(ops, bs, tol=1e-8, precision="fp64", maxit=None, inner_tol=None,
 
 
 class MaxwellOperator:
-    """Device-resident A = Cₘ μ⁻¹ Cₑ − ω² Pε (model.jl:235-237) with its solver.
+    """Device-resident A of create_A (model.jl:225-246) with its solver:
+    ft = EE, A = Cₘ μ⁻¹ Cₑ − ω² Pε; ft = HH, A = Cₑ ε⁻¹ Cₘ − ω² Pμ (built as
+    the identical EE operator of the model with every boundft flipped and ε,
+    μ swapped).  Any boundft.
 
     solve() is the user's `A \\ b` of the reference workflow (iterative:
     Jacobi-scaled symmetric QMR; precision "fp64" or "mixed" = FP32 inner

@@ -44,7 +44,7 @@ CASES = {
 
 
 def make(pkg, case):
-    c = CASES[case]
+    c = CASES[case] if case in CASES else HHCASES[case]
     N = c["N"]
     lp = nonuniform(N, c["nu"]) if "nu" in c else [np.arange(n + 1.0) for n in N]
     g = pkg.Grid(lp, c["bloch"])
@@ -267,6 +267,82 @@ def test_apply_matches_csc(case, full, mu, pkg, O):
     for r in range(3):
         ref = A @ X[r]
         assert np.linalg.norm(Y[r] - ref) <= 1e-12 * np.linalg.norm(ref)
+    op.close()
+
+
+HHCASES = {
+    "boundft_HH": CASES["boundft_HH"],
+    "hh_pml": dict(N=(5, 6, 4), npml=((1, 1, 0), (1, 0, 1)), om=0.7, bloch=(False,) * 3, bft=(1, 0, 1), nu=3),
+    "hh_all_bloch": dict(N=(4, 5, 3), npml=((0, 0, 1), (0, 0, 1)), om=0.6, bloch=(True, True, False),
+                         kb=(0.2, 0.0, 0.0), bft=(1, 1, 1), nu=4),
+}
+
+
+@pytest.mark.parametrize("case", list(HHCASES))
+@pytest.mark.parametrize("mu", [False, True])
+def test_apply_boundft_hh(case, mu, pkg, O):
+    """boundft = HH on some axes: the matrix-free EE operator (direct kernel,
+    difference directions per axis) against the oracle's CSC; off-diagonal ε
+    there is refused."""
+    m = make(pkg, case)
+    eps, muu = rand_params(m, 17, mu=mu)
+    om = 0.55
+    say("apply boundft HH", case, mu)
+    A = O.to_scipy(O.create_A(spec_of(m), om, eps, muu))
+    op = pkg.MaxwellOperator(m, om, εarr=eps, μarr=muu)
+    n = 3 * m.grid.M
+    X = np.random.default_rng(3).normal(size=(2, n)) + 1j * np.random.default_rng(4).normal(size=(2, n))
+    Y = op.apply(X)
+    for r in range(2):
+        ref = A @ X[r]
+        assert np.linalg.norm(Y[r] - ref) <= 1e-12 * np.linalg.norm(ref)
+    op.close()
+    epsf, _ = rand_params(m, 17, full=True)
+    if any(b == 1 for b in m.boundft):
+        with pytest.raises(pkg.MfdUnsupportedError):
+            pkg.MaxwellOperator(m, om, εarr=epsf, μarr=muu)
+
+
+@pytest.mark.parametrize("case", ["uniform_pml", "hh_pml", "boundft_HH"])
+def test_operator_ft_hh(case, pkg, O):
+    """The HH formulation Cₑ (Pε \\ Cₘ) − ω² Pμ as a device operator (the dual
+    EE operator of the flipped model) against the oracle's HH CSC, and its
+    solve against the direct solve; the EE-only field conversions refuse it."""
+    m = make(pkg, case)
+    muu, eps = rand_params(m, 19, mu=True)   # ε diagonal (the divisor), μ ≠ 1
+    om = 0.5 + 0.03j
+    say("operator ft=HH", case)
+    A = O.to_scipy(O.create_A(spec_of(m), om, eps, muu, ft=1))
+    op = pkg.MaxwellOperator(m, om, ft=pkg.HH, εarr=eps, μarr=muu)
+    n = 3 * m.grid.M
+    X = np.random.default_rng(5).normal(size=n) + 1j * np.random.default_rng(6).normal(size=n)
+    ref = A @ X
+    assert np.linalg.norm(op.apply(X) - ref) <= 1e-12 * np.linalg.norm(ref)
+    b = np.random.default_rng(7).normal(size=n) + 1j * np.random.default_rng(8).normal(size=n)
+    h, st = op.solve(b, tol=1e-9)
+    xd = sla.spsolve(A.tocsc(), b)
+    assert st.converged and np.linalg.norm(b - A @ h) / np.linalg.norm(b) <= 1.05e-9
+    assert np.linalg.norm(h - xd) / np.linalg.norm(xd) <= 1e-6
+    with pytest.raises(pkg.MfdUnsupportedError):
+        op.h_from_e(h)
+    op.close()
+
+
+@pytest.mark.parametrize("case", list(HHCASES))
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_boundft_hh(case, precision, pkg, O):
+    """EE solve with boundft = HH on some axes against the direct solve."""
+    m = make(pkg, case)
+    eps, muu = rand_params(m, 21)
+    om = 0.45 + 0.05j
+    A = O.to_scipy(O.create_A(spec_of(m), om, eps, muu))
+    op = pkg.MaxwellOperator(m, om, εarr=eps, μarr=muu)
+    b = np.random.default_rng(9).normal(size=A.shape[0]) + 0j
+    say("solve boundft HH", case, precision)
+    e, st = op.solve(b, tol=1e-9, precision=precision)
+    xd = sla.spsolve(A.tocsc(), b)
+    assert st.converged and np.linalg.norm(b - A @ e) / np.linalg.norm(b) <= 1.05e-9
+    assert np.linalg.norm(e - xd) / np.linalg.norm(xd) <= 1e-6
     op.close()
 
 
