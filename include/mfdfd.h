@@ -149,8 +149,7 @@ int mfd_A_csc(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_
               const mfd_c64 *mu_arr, int64_t *colptr, int64_t *rowval, mfd_c64 *nzval);
 
 /* create_b (model.jl:251-274): b = −Cₘ (Pμ \ jₘ) − iω jₑ for ft = EE (uses
- * mu_arr), b = Cₑ (Pε \ jₑ) − iω jₘ for ft = HH (uses eps_arr); boundft = EE
- * on every axis. */
+ * mu_arr), b = Cₑ (Pε \ jₑ) − iω jₘ for ft = HH (uses eps_arr); any boundft. */
 int mfd_create_b(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr,
                  const mfd_c64 *mu_arr, const mfd_c64 *je, const mfd_c64 *jm, mfd_c64 *b);
 
@@ -213,7 +212,7 @@ int mfd_solve_batch_dev(int count, mfd_op *const *ops, const void *const *b, voi
                         const mfd_solve_opts *o, mfd_solve_stats *st);
 /* h_from_e (model.jl:276-279): h = (i/ω) Pμ⁻¹ (Cₑ e + jₘ) with the operator's
  * grid, μ and ω; e, jm, h host DOF vectors of 3M values (jm may be null).
- * Needs an EE operator with boundft = EE on every axis (as e_from_h does). */
+ * Needs an EE-formulation operator (as e_from_h does). */
 int mfd_h_from_e(mfd_op *op, const mfd_c64 *e, const mfd_c64 *jm, mfd_c64 *h);
 /* e_from_h (model.jl:281-284): e = (−i/ω) Pε⁻¹ (Cₘ h − jₑ); diagonal ε only
  * (MFD_ERR_UNSUPPORTED for an operator with off-diagonal ε); je may be null. */

@@ -274,8 +274,6 @@ int mfd_create_b(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const m
                  const mfd_c64 *mu_arr, const mfd_c64 *je, const mfd_c64 *jm, mfd_c64 *b) {
     CHECK_CTX(ctx);
     MFD_TRY(ft_check(ft));
-    for (int a = 0; a < 3; ++a)
-        if (g->boundft[a] != MFD_EE) { set_error("create_b: boundft must be EE on every axis"); return MFD_ERR_UNSUPPORTED; }
     DevGrid G;
     MFD_TRY(G.create(g, ctx->stream));
     return create_b_impl(ctx, G, ft, omega, ft == MFD_EE ? mu_arr : eps_arr, je, jm, b);

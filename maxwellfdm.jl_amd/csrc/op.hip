@@ -2451,18 +2451,23 @@ __global__ __launch_bounds__(BS) void k_h_from_e(DGrid<double> g, OpData<double>
     }
 }
 
-// H_c of a given H vector at (i,j,k) with one index possibly −1: the ghost
-// below the domain is the Bloch image (times e^{+ikL}⁻¹) or zero
+// H_c of a given H vector at (i,j,k) with one index possibly −1 or N: the
+// ghost is the Bloch image (times the phase) or zero
 __device__ inline c64 ldH(const DGrid<double> &g, const c64 *__restrict__ h, int c, int i, int j, int k) {
     c64 ph = cone<double>();
     int id[3] = {i, j, k};
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < 3; ++a) {
         if (id[a] < 0) {
             if (!g.bloch[a]) return czero<double>();
             id[a] += g.N[a];
             ph = ph * g.phinv[a];
+        } else if (id[a] >= g.N[a]) {
+            if (!g.bloch[a]) return czero<double>();
+            id[a] -= g.N[a];
+            ph = ph * g.ph[a];
         }
+    }
     int64_t p = id[0] + (int64_t)g.N[0] * (id[1] + (int64_t)g.N[1] * id[2]);
     return h[dofidx(g, c, p)] * ph;
 }
@@ -2477,12 +2482,20 @@ __global__ __launch_bounds__(BS) void k_e_from_h(DGrid<double> g, OpData<double>
 #pragma unroll
         for (int u = 0; u < 3; ++u) {
             int v = u == 0 ? 1 : (u == 1 ? 2 : 0), w = u == 0 ? 2 : (u == 1 ? 0 : 1);
-            int iv[3] = {i, j, k}, iw_[3] = {i, j, k};
-            iv[v] -= 1;
-            iw_[w] -= 1;
-            // (Cₘ h)_u = ∂_v h_w − ∂_w h_v (backward differences)
-            c64 cc = g.sdlinv[PRIM][v][id[v]] * (ldH(g, h, w, i, j, k) - ldH(g, h, w, iv[0], iv[1], iv[2])) -
-                     g.sdlinv[PRIM][w][id[w]] * (ldH(g, h, v, i, j, k) - ldH(g, h, v, iw_[0], iw_[1], iw_[2]));
+            // ∂_a h_c in Cₘ's direction along a (backward where boundft is EE, forward where HH)
+            auto dif = [&](int a, int c) -> c64 {
+                int i1[3] = {i, j, k};
+                const c64 h0 = ldH(g, h, c, i, j, k);
+                if (g.bft[a] == EE) {
+                    i1[a] -= 1;
+                    return h0 - ldH(g, h, c, i1[0], i1[1], i1[2]);
+                }
+                i1[a] += 1;
+                return ldH(g, h, c, i1[0], i1[1], i1[2]) - h0;
+            };
+            // (Cₘ h)_u = ∂_v h_w − ∂_w h_v
+            c64 cc = g.sdlinv[g0type(g.bft, EE, v)][v][id[v]] * dif(v, w) -
+                     g.sdlinv[g0type(g.bft, EE, w)][w][id[w]] * dif(w, v);
             int64_t d = dofidx(g, u, p);
             if (je) cc = cc - je[d];
             e[d] = miw * cdiv(cc, od.epsd[3 * p + u]);
@@ -2492,8 +2505,8 @@ __global__ __launch_bounds__(BS) void k_e_from_h(DGrid<double> g, OpData<double>
 
 // which = 0: out = h_from_e(in, src = jₘ); which = 1: out = e_from_h(in, src = jₑ)
 int field_convert_impl(mfd_op *op, int which, const mfd_c64 *in, const mfd_c64 *src, mfd_c64 *out) {
-    if (op->ft != MFD_EE || !op->ee_all) {
-        set_error("h_from_e/e_from_h: operator of an EE model with boundft = EE on every axis");
+    if (op->ft != MFD_EE) {
+        set_error("h_from_e/e_from_h: operator of the EE formulation");
         return MFD_ERR_UNSUPPORTED;
     }
     if (which == 1 && op->full) { set_error("e_from_h: only diagonal ε is supported (Pε \\ is a sparse solve otherwise)"); return MFD_ERR_UNSUPPORTED; }
@@ -2552,70 +2565,53 @@ int adjoint_grad_impl(mfd_op *op, const mfd_c64 *e, const mfd_c64 *lam, double *
 // ------------------------------------------------------------- create_b
 
 namespace mfd {
-// b = −Cₘ (Pμ \ jₘ) − iω jₑ  (model.jl:262-265), one thread per cell
-__global__ __launch_bounds__(BS) void k_create_b(DGrid<double> g, const c64 *__restrict__ mu, const c64 *__restrict__ je,
-                                                 const c64 *__restrict__ jm, c64 iw, int omz, c64 *__restrict__ b) {
-    int64_t p = blockIdx.x * (int64_t)BS + threadIdx.x;
-    if (p >= g.M) return;
-    int i, j, k;
-    cell3(g, p, i, j, k);
-    int id[3] = {i, j, k};
-    auto h = [&](int c, int a) -> c64 {   // (jₘ/μ)_c at p − â (a = -1: at p)
-        int r[3] = {i, j, k};
-        c64 ph = mk<double>(1, 0);
-        if (a >= 0) {
-            r[a] -= 1;
-            if (r[a] < 0) {
-                if (!g.bloch[a]) return mk<double>(0, 0);
-                r[a] += g.N[a];
-                ph = g.phinv[a];
-            }
+// value of f_c / p_cc (f a current, p the divided parameter) at cell
+// p + off·â (off ∈ {−1, 0, +1}): a Bloch wrap carries the phase, a non-Bloch
+// ghost is zero
+__device__ inline c64 div_at(const DGrid<double> &g, const c64 *__restrict__ f, const c64 *__restrict__ par, int c,
+                             int i, int j, int k, int a, int off) {
+    int r[3] = {i, j, k};
+    c64 ph = mk<double>(1, 0);
+    if (off) {
+        r[a] += off;
+        if (r[a] < 0 || r[a] >= g.N[a]) {
+            if (!g.bloch[a]) return mk<double>(0, 0);
+            r[a] -= off * g.N[a];
+            ph = off > 0 ? g.ph[a] : g.phinv[a];
         }
-        int64_t q = r[0] + (int64_t)g.N[0] * (r[1] + (int64_t)g.N[1] * r[2]);
-        c64 v = cdiv(jm[dofidx(g, c, q)], mu[q + g.M * (c + 3 * c)]);
-        return v * ph;
-    };
-    for (int u = 0; u < 3; ++u) {
-        int v = u == 0 ? 1 : (u == 1 ? 2 : 0), w = u == 0 ? 2 : (u == 1 ? 0 : 1);
-        c64 y = g.sdlinv[PRIM][v][id[v]] * (h(w, -1) - h(w, v)) - g.sdlinv[PRIM][w][id[w]] * (h(v, -1) - h(v, w));
-        int64_t e = dofidx(g, u, p);
-        c64 out = -y;
-        if (!omz) out = out - iw * je[e];
-        b[e] = out;
     }
+    int64_t q = r[0] + (int64_t)g.N[0] * (r[1] + (int64_t)g.N[1] * r[2]);
+    return cdiv(f[dofidx(g, c, q)], par[q + g.M * (c + 3 * c)]) * ph;
 }
 
-// HH: b = Cₑ (Pε \ jₑ) − iω jₘ  (model.jl:266-268), one thread per cell
-__global__ __launch_bounds__(BS) void k_create_b_hh(DGrid<double> g, const c64 *__restrict__ eps,
-                                                    const c64 *__restrict__ je, const c64 *__restrict__ jm, c64 iw,
-                                                    int omz, c64 *__restrict__ b) {
+// b = −Cₘ (Pμ \ jₘ) − iω jₑ (ft = EE, model.jl:263) or b = Cₑ (Pε \ jₑ) − iω jₘ
+// (ft = HH, model.jl:266-268), one thread per cell.  Cₑ differences run
+// forward along an axis whose boundft is EE and backward where it is HH, Cₘ
+// the other way, each with the ∆l of its output grid.
+__global__ __launch_bounds__(BS) void k_create_b(DGrid<double> g, int ft, const c64 *__restrict__ par,
+                                                 const c64 *__restrict__ je, const c64 *__restrict__ jm, c64 iw,
+                                                 int omz, c64 *__restrict__ b) {
     int64_t p = blockIdx.x * (int64_t)BS + threadIdx.x;
     if (p >= g.M) return;
     int i, j, k;
     cell3(g, p, i, j, k);
     int id[3] = {i, j, k};
-    auto x = [&](int c, int a) -> c64 {   // (jₑ/ε)_c at p + â (a = -1: at p)
-        int r[3] = {i, j, k};
-        c64 ph = mk<double>(1, 0);
-        if (a >= 0) {
-            r[a] += 1;
-            if (r[a] >= g.N[a]) {
-                if (!g.bloch[a]) return mk<double>(0, 0);
-                r[a] -= g.N[a];
-                ph = g.ph[a];
-            }
-        }
-        int64_t q = r[0] + (int64_t)g.N[0] * (r[1] + (int64_t)g.N[1] * r[2]);
-        c64 v = cdiv(je[dofidx(g, c, q)], eps[q + g.M * (c + 3 * c)]);
-        return v * ph;
+    const c64 *f = ft == EE ? jm : je;   // the current under the curl
+    const c64 *jd = ft == EE ? je : jm;  // the current in the −iω term
+    // difference of f_c/p along a in the curl's direction
+    auto dif = [&](int c, int a) -> c64 {
+        const bool fwd = ft == EE ? g.bft[a] == HH : g.bft[a] == EE;   // Cₘ for EE, Cₑ for HH
+        const c64 y0 = div_at(g, f, par, c, i, j, k, a, 0);
+        return fwd ? div_at(g, f, par, c, i, j, k, a, 1) - y0 : y0 - div_at(g, f, par, c, i, j, k, a, -1);
     };
+    const int gout = ft == EE ? EE : HH;   // the output field's grid (E for Cₘ, H for Cₑ)
     for (int u = 0; u < 3; ++u) {
         int v = u == 0 ? 1 : (u == 1 ? 2 : 0), w = u == 0 ? 2 : (u == 1 ? 0 : 1);
-        // (Cₑ y)_u = ∂_v y_w − ∂_w y_v (forward differences)
-        c64 y = g.sdlinv[DUAL][v][id[v]] * (x(w, v) - x(w, -1)) - g.sdlinv[DUAL][w][id[w]] * (x(v, w) - x(v, -1));
+        c64 y = g.sdlinv[g0type(g.bft, gout, v)][v][id[v]] * dif(w, v) -
+                g.sdlinv[g0type(g.bft, gout, w)][w][id[w]] * dif(v, w);
         int64_t e = dofidx(g, u, p);
-        c64 out = y;
-        if (!omz) out = out - iw * jm[e];
+        c64 out = ft == EE ? -y : y;
+        if (!omz) out = out - iw * jd[e];
         b[e] = out;
     }
 }
@@ -2643,8 +2639,7 @@ int create_b_impl(mfd_ctx *ctx, const mfd::DevGrid &G, int ft, mfd_c64 omega, co
     if (!rc) {
         c64 iw = mk<double>(-omega.im, omega.re);
         int omz = omega.re == 0.0 && omega.im == 0.0;
-        if (ft == EE) k_create_b<<<(unsigned)((M + BS - 1) / BS), BS, 0, s>>>(G.d64, (const c64 *)U.p, dje, djm, iw, omz, db);
-        else k_create_b_hh<<<(unsigned)((M + BS - 1) / BS), BS, 0, s>>>(G.d64, (const c64 *)U.p, dje, djm, iw, omz, db);
+        k_create_b<<<(unsigned)((M + BS - 1) / BS), BS, 0, s>>>(G.d64, ft, (const c64 *)U.p, dje, djm, iw, omz, db);
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(b, db, sizeof(c64) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess)

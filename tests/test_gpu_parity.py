@@ -163,16 +163,13 @@ def test_A_rejects_nondiagonal_divisor(pkg):
         pkg.A_csc(m, 0.5, eps2, mu, ft=pkg.HH)
     with pytest.raises(pkg.MfdUnsupportedError):
         pkg.create_b(pkg.HH, 0.5, m, np.zeros(3 * m.grid.M, complex), np.zeros(3 * m.grid.M, complex), εarr=eps2)
-    mb = make(pkg, "boundft_HH")
-    with pytest.raises(pkg.MfdUnsupportedError):
-        pkg.create_b(pkg.EE, 0.5, mb, np.zeros(3 * mb.grid.M, complex), np.zeros(3 * mb.grid.M, complex))
     m2 = make(pkg, "uniform_pml")
     pkg.set_ωpml(m2, 0.0)
     with pytest.raises(pkg.MfdArgumentError):
         pkg.create_stretched_dls(m2)
 
 
-@pytest.mark.parametrize("case", ["uniform_pml", "mixed_pml_bloch", "cmplast"])
+@pytest.mark.parametrize("case", ["uniform_pml", "mixed_pml_bloch", "cmplast", "boundft_HH", "hh_pml", "hh_all_bloch"])
 def test_create_b(case, pkg, O):
     m = make(pkg, case)
     s = spec_of(m)
@@ -346,7 +343,8 @@ def test_solve_boundft_hh(case, precision, pkg, O):
     op.close()
 
 
-@pytest.mark.parametrize("case", ["uniform_pml", "nonuniform_bloch", "mixed_pml_bloch", "cmplast", "wide"])
+@pytest.mark.parametrize("case", ["uniform_pml", "nonuniform_bloch", "mixed_pml_bloch", "cmplast", "wide",
+                                  "boundft_HH", "hh_pml", "hh_all_bloch"])
 @pytest.mark.parametrize("mu", [False, True])
 def test_h_from_e_e_from_h(case, mu, pkg, O):
     """h_from_e / e_from_h (model.jl:276-284) against the oracle's Cₑ, Cₘ, Pε, Pμ
