@@ -78,7 +78,8 @@ __device__ inline cx<R> ldX(const DGrid<R> &g, const OpData<R> &od, const cx<R> 
     }
     int64_t p = id[0] + (int64_t)g.N[0] * (id[1] + (int64_t)g.N[1] * id[2]);
     cx<R> v = x[dofidx(g, c, p)];
-    if (MODE == MODE_P) v = v * (od.q[3 * p + c] * tfac(g, c, id[0], id[1], id[2], true));
+    if (MODE == MODE_P)   // the stored q·T⁻¹ where the operator has it (FP64 data)
+        v = v * (od.qs ? od.qs[3 * p + c] : od.q[3 * p + c] * tfac(g, c, id[0], id[1], id[2], true));
     return wr ? v * ph : v;
 }
 
@@ -270,14 +271,55 @@ __global__ __launch_bounds__(BS) void k_fin_resid(const double *part, int nb, do
 // y = A x (MODE_A) or y = P x (MODE_P) for the RHS blockIdx.y.
 // EPI_STORE: y = op(x).  EPI_RESID: y = b − op(x), reduce ||y||² into rn.
 // EPI_DOT: y = op(x), reduce xᵀy into the solver scalars (σ, α).
-template <typename R, int MODE, bool OFFD, bool MU, int EPI>
+// H value from the scratch of k_hpass at (i,j,k) with one index possibly −1
+// or N: the Bloch image times the phase, or zero past a non-Bloch end
+template <typename R>
+__device__ inline cx<R> ldHb(const DGrid<R> &g, const cx<R> *__restrict__ hb, int c, int i, int j, int k) {
+    cx<R> ph = cone<R>();
+    bool wr = false;
+    int id[3] = {i, j, k};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (id[a] < 0) {
+            if (!g.bloch[a]) return czero<R>();
+            id[a] += g.N[a]; ph = ph * g.phinv[a]; wr = true;
+        } else if (id[a] >= g.N[a]) {
+            if (!g.bloch[a]) return czero<R>();
+            id[a] -= g.N[a]; ph = ph * g.ph[a]; wr = true;
+        }
+    }
+    cx<R> v = hb[dofidx(g, c, id[0] + (int64_t)g.N[0] * (id[1] + (int64_t)g.N[1] * id[2]))];
+    return wr ? v * ph : v;
+}
+
+// first pass of the two-pass direct kernel: H = μ⁻¹ Cₑ X at every cell, once
+template <typename R, int MODE, bool MU>
+__global__ __launch_bounds__(BS) void k_hpass(DGrid<R> g, OpData<R> od, const cx<R> *__restrict__ xall,
+                                              cx<R> *__restrict__ hall, int64_t vs, const Scal *scal, int epi_dot) {
+    const int rhs = blockIdx.y;
+    if (epi_dot && (scal[rhs].done || scal[rhs].conv)) return;
+    const cx<R> *x = xall + rhs * vs;
+    cx<R> *hb = hall + rhs * vs;
+    for (int64_t p = blockIdx.x * (int64_t)BS + threadIdx.x; p < g.M; p += (int64_t)gridDim.x * BS) {
+        int i, j, k;
+        cell3(g, p, i, j, k);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) hb[dofidx(g, c, p)] = Hval<R, MODE, MU>(g, od, x, c, i, j, k);
+    }
+}
+
+// HB: H comes from k_hpass's scratch (hall) instead of being recomputed at p
+// and its six neighbours
+template <typename R, int MODE, bool OFFD, bool MU, int EPI, bool HB = false>
 __global__ __launch_bounds__(BS) void k_apply(DGrid<R> g, OpData<R> od, const cx<R> *__restrict__ xall,
                                               cx<R> *__restrict__ yall, const c64 *__restrict__ ball, int64_t vs,
                                               double *part, const Scal *scal,
-                                              const int *__restrict__ mask = nullptr) {
+                                              const int *__restrict__ mask = nullptr,
+                                              const cx<R> *__restrict__ hall = nullptr) {
     const int rhs = blockIdx.y;
     const cx<R> *x = xall + rhs * vs;
     cx<R> *y = yall + rhs * vs;
+    const cx<R> *hb = HB ? hall + rhs * vs : nullptr;
     bool skip = false;
     if (EPI == EPI_RESID && mask && !mask[rhs]) return;
     if (EPI == EPI_DOT) {
@@ -293,7 +335,7 @@ __global__ __launch_bounds__(BS) void k_apply(DGrid<R> g, OpData<R> od, const cx
             // along a takes (p − â along an EE axis, p + â along an HH axis)
             cx<R> H[3], Hm[3][3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) H[c] = Hval<R, MODE, MU>(g, od, x, c, i, j, k);
+            for (int c = 0; c < 3; ++c) H[c] = HB ? hb[dofidx(g, c, p)] : Hval<R, MODE, MU>(g, od, x, c, i, j, k);
 #pragma unroll
             for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -301,7 +343,8 @@ __global__ __launch_bounds__(BS) void k_apply(DGrid<R> g, OpData<R> od, const cx
                     if (a == c) continue;
                     int id[3] = {i, j, k};
                     id[a] += g.bft[a] == EE ? -1 : 1;
-                    Hm[c][a] = Hval<R, MODE, MU>(g, od, x, c, id[0], id[1], id[2]);
+                    Hm[c][a] = HB ? ldHb(g, hb, c, id[0], id[1], id[2])
+                                  : Hval<R, MODE, MU>(g, od, x, c, id[0], id[1], id[2]);
                 }
             int id[3] = {i, j, k};
 #pragma unroll
@@ -815,9 +858,32 @@ void Workspace::release() {
     nrhs = 0;
 }
 
+// scratch of the two-pass direct kernel for nrhs right-hand sides (operators
+// the tiled stencils do not serve); allocated outside any graph capture
+static int hbuf_ensure(mfd_op *op, int64_t nrhs) {
+    const bool tiled_ok = !op->has_mu && op->ee_all;
+    if (tiled_ok && op->tile64[0] > 0 && op->tile32[0] > 0) return MFD_OK;
+    const size_t need = sizeof(c64) * 3 * (size_t)op->G.h.M * (size_t)nrhs;
+    if (op->hbuf_bytes >= need) return MFD_OK;
+    if (op->hbuf) {
+        MFD_HIP(hipStreamSynchronize(op->stream));
+        hipFree(op->hbuf);
+        op->hbuf = nullptr;
+        op->hbuf_bytes = 0;
+    }
+    if (hipMalloc(&op->hbuf, need) != hipSuccess) {
+        (void)hipGetLastError();   // clear it: the single-pass kernel serves without the scratch
+        op->hbuf = nullptr;
+        return MFD_OK;
+    }
+    op->hbuf_bytes = need;
+    return MFD_OK;
+}
+
 static int ws_ensure(mfd_op *op, int64_t nrhs, int prec) {
     Workspace &w = op->ws;
     int64_t n = 3 * op->G.h.M;
+    MFD_TRY(hbuf_ensure(op, nrhs));
     if (w.buf && w.nrhs == nrhs && w.prec == prec) return MFD_OK;
     w.release();
     // Krylov vectors are sized for FP64 in both modes: a mixed solve whose
@@ -907,13 +973,27 @@ static int launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int6
         return nx * ny * nch;
     }
     dim3 grid(ngrid(g.M), (unsigned)nrhs);
-    if (op->full) {
-        if (op->has_mu) k_apply<R, MODE, true, true, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, scal, mask);
-        else k_apply<R, MODE, true, false, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, scal, mask);
-    } else {
-        if (op->has_mu) k_apply<R, MODE, false, true, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, scal, mask);
-        else k_apply<R, MODE, false, false, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, scal, mask);
+    // two passes when the scratch is there: H = μ⁻¹ Cₑ X once per cell, then
+    // E from H (instead of recomputing H at each cell and its six neighbours)
+    const bool two = op->hbuf && op->hbuf_bytes >= sizeof(cx<R>) * (size_t)vs * (size_t)nrhs;
+    cx<R> *hb = (cx<R> *)op->hbuf;
+    if (two) {
+        if (op->has_mu) k_hpass<R, MODE, true><<<grid, BS, 0, s>>>(g, od, x, hb, vs, scal, EPI == EPI_DOT);
+        else k_hpass<R, MODE, false><<<grid, BS, 0, s>>>(g, od, x, hb, vs, scal, EPI == EPI_DOT);
     }
+#define MFD_DIRECT(OFF, MU_)                                                                                   \
+    do {                                                                                                       \
+        if (two) k_apply<R, MODE, OFF, MU_, EPI, true><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, scal, mask, hb); \
+        else k_apply<R, MODE, OFF, MU_, EPI><<<grid, BS, 0, s>>>(g, od, x, y, b, vs, part, scal, mask);          \
+    } while (0)
+    if (op->full) {
+        if (op->has_mu) MFD_DIRECT(true, true);
+        else MFD_DIRECT(true, false);
+    } else {
+        if (op->has_mu) MFD_DIRECT(false, true);
+        else MFD_DIRECT(false, false);
+    }
+#undef MFD_DIRECT
     if (after_main) hipEventRecord(after_main, s);
     if (finish) launch_finish<EPI>(s, (int)grid.x, nrhs, part, scal, rn, mask);
     return (int)grid.x;
@@ -1085,6 +1165,7 @@ int op_destroy_impl(mfd_op *op) {
     hipStreamSynchronize(op->stream);
     op->ws.release();
     if (op->buf) hipFree(op->buf);
+    if (op->hbuf) hipFree(op->hbuf);
     if (op->ev0) hipEventDestroy(op->ev0);
     if (op->ev1) hipEventDestroy(op->ev1);
     delete op;
@@ -1113,6 +1194,7 @@ int op_set_graddiv_impl(mfd_op *op, int enable) {
 
 int op_apply_dev_impl(mfd_op *op, const void *x, void *y, int64_t nrhs, int prec) {
     if (nrhs < 1 || nrhs > 65535) { set_error("nrhs out of range"); return MFD_ERR_ARG; }
+    MFD_TRY(hbuf_ensure(op, nrhs));
     if (prec == MFD_FP64) launch_apply<double, MODE_A, EPI_STORE>(op, (const c64 *)x, (c64 *)y, nullptr, nrhs, nullptr, nullptr, nullptr);
     else launch_apply<float, MODE_A, EPI_STORE>(op, (const c32 *)x, (c32 *)y, nullptr, nrhs, nullptr, nullptr, nullptr);
     MFD_HIP(hipGetLastError());
@@ -1139,6 +1221,7 @@ int op_apply_impl(mfd_op *op, const mfd_c64 *x, mfd_c64 *y, int64_t nrhs) {
 
 // average ms of one stencil launch (EPI_STORE): MODE_A or MODE_P, FP64 or FP32 vectors
 static int time_apply(mfd_op *op, int64_t nrhs, int prec, int mode, int iters, double *ms) {
+    MFD_TRY(hbuf_ensure(op, nrhs));
     int64_t n = 3 * op->G.h.M;
     size_t es = prec == MFD_FP64 ? sizeof(c64) : sizeof(c32);
     void *dx = nullptr, *dy = nullptr;

@@ -77,5 +77,7 @@ struct mfd_op {
     int gd = 0;               // grad-div augmented inner operator in use
     int gd_ok = 0;            // eligible (diagonal ε, μ = 1)
     int ft = 0;               // formulation the caller asked for (MFD_EE, or MFD_HH served as the dual EE operator)
+    void *hbuf = nullptr;     // H = μ⁻¹ Cₑ x scratch of the two-pass direct kernel (3M values per RHS)
+    size_t hbuf_bytes = 0;
     int ee_all = 1;           // boundft = EE on every axis (the tiled stencils and field conversions need it)
 };
