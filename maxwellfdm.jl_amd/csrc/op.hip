@@ -496,6 +496,10 @@ __global__ __launch_bounds__(BS) void k_upd_r_dot(cx<R> *__restrict__ rall, cons
             S.conv = 1;
             S.alpha_re = S.alpha_im = 0;
             S.apend_re = S.apend_im = 0;
+            // k_fin_r skips a converged RHS, so k_upd_dzq would otherwise
+            // apply the previous step's coefficients once more: d = 0, z kept
+            S.c2t2 = 0;
+            S.c2a_re = S.c2a_im = 0;
         }
         return;
     }
