@@ -1731,7 +1731,7 @@ void solve_opts_default(mfd_solve_opts *o) {
 template <typename R>
 static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double rel_target, const int *active_d,
                      long long budget, long long *iters_out, const void *b = nullptr, const void *e = nullptr,
-                     const double *bn = nullptr, bool *stalled = nullptr) {
+                     const double *bn = nullptr, bool *stalled = nullptr, double tol = 0, double cyc_res = 0) {
     if (stalled) *stalled = false;
     Workspace &w = op->ws;
     hipStream_t s = op->stream;
@@ -1764,6 +1764,15 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
     const int stall_max = 2;
     double best_probe = 1e300;
     int stall = 0;
+    // FP64 cycles (given tol and the cycle's starting residual cyc_res) end on
+    // the true residual: the recurrence estimate τ runs to the cycle target
+    // (0.1 tol / cyc_res, a 10x margin for the gap between τ and the true
+    // residual), but once every running RHS has τ/‖c‖ · cyc_res ≤ 3 tol the
+    // true FP64 residual is probed every `probe64` steps and the cycle ends as
+    // soon as it is ≤ 0.9 tol for every RHS (the outer loop recomputes it the
+    // same way).  A probe costs about one step.
+    const long long probe64 = std::max<long long>(chunk, 256) / chunk * chunk;
+    long long next64 = 0;
     while (true) {
         if (use_graph) MFD_HIP(hipGraphLaunch(w.graph, s));
         else
@@ -1780,9 +1789,23 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
         }
         *iters_out = mx;
         if (all || mx >= budget) break;
+        bool probe_now = false;
+        if (sizeof(R) == 4) probe_now = b && e && done_it % probe == 0;
+        else if (b && e && tol > 0 && cyc_res > 0 && done_it >= next64) {
+            bool near = true;
+            for (int64_t r = 0; r < nrhs; ++r) {
+                const Scal &S = w.scal_h[r];
+                if (!S.conv && S.tau > 3.0 * (tol / cyc_res) * S.cn) near = false;
+            }
+            if (near) {
+                probe_now = true;
+                next64 = done_it + probe64;
+            }
+        }
         // FP32 cycles: probe the true FP64 residual every `probe` iterations and
-        // end the cycle once it stops improving (2x per probe window)
-        if (sizeof(R) == 4 && b && e && done_it % probe == 0) {
+        // end the cycle once it stops improving (2x per probe window); FP64
+        // cycles: end once it meets the tolerance
+        if (probe_now) {
             dim3 gv2(ngrid(n), (unsigned)nrhs);
             k_probe_x<R><<<gv2, BS, 0, s>>>(op->G.d64, op->d64, (const c64 *)e, (const cx<R> *)w.z, (c64 *)w.et, n, n);
             size_t ps = (size_t)nrhs * NPART * 4;
@@ -1793,6 +1816,10 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
             double worst = 0;
             for (int64_t r = 0; r < nrhs; ++r) worst = std::max(worst, bn[r] > 0 ? sqrt(w.rn_h[r] / bn[r]) : 0.0);
             if (getenv("MFD_SOLVE_TRACE")) fprintf(stderr, "[mfd probe] it %lld true residual %.3e\n", mx, worst);
+            if (sizeof(R) == 8) {
+                if (worst <= 0.9 * tol) break;
+                continue;
+            }
             if (worst > 0.5 * best_probe) {
                 if (++stall >= stall_max) {
                     if (stalled) *stalled = true;
@@ -2146,7 +2173,9 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
         if (method == MFD_BICGSTAB) rc = run_inner_bicgstab(op, nrhs, &o, target, active_d, o.maxit - total_it, &it);
         else if (cyc_prec == MFD_MIXED)
             rc = run_inner<float>(op, nrhs, &o, target, active_d, o.maxit - total_it, &it, b, e, bn.data(), &stalled32);
-        else rc = run_inner<double>(op, nrhs, &o, target, active_d, o.maxit - total_it, &it);
+        else
+            rc = run_inner<double>(op, nrhs, &o, target, active_d, o.maxit - total_it, &it, b, e, bn.data(), nullptr,
+                                   op->gd ? 0.0 : o.tol, worst);
         if (rc) return rc;
         total_it += it;
         if (cyc_prec == MFD_MIXED) it32 += it;
