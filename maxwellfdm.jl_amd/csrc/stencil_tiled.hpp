@@ -204,10 +204,11 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
     // owner's output factors (x from the column, y from the row)
     const int io = own ? i : 0, jo = own ? j : 0;
     const c64 exo = g.sdlinv[PRIM][0][io], eyo = g.sdlinv[PRIM][1][jo];
-    c64 Txy[3];
+    // x-y part of T² (T = S_E^{1/2}: the product of the stretched ∆l)
+    c64 T2xy[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-        Txy[c] = g.ssq[c == 0 ? DUAL : PRIM][0][io] * g.ssq[c == 1 ? DUAL : PRIM][1][jo];
+        T2xy[c] = g.sdl[c == 0 ? DUAL : PRIM][0][io] * g.sdl[c == 1 ? DUAL : PRIM][1][jo];
     const c64 tmx = axis_term(g, 0, io), tmy = axis_term(g, 1, jo);
     const c64 gdx = g.sdlinv[DUAL][0][io], gdy = g.sdlinv[DUAL][1][jo];
 
@@ -431,7 +432,7 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
             // wait for the prefetch)
             const c64 az_k = ldu(g.sdlinv[DUAL][2], k), ez_k = ldu(g.sdlinv[PRIM][2], k);
             const c64 tzm = axis_term_u(g, k);
-            const c64 szD = ldu(g.ssq[DUAL][2], k), szP = ldu(g.ssq[PRIM][2], k);
+            const c64 sdD = ldu(g.sdl[DUAL][2], k), sdP = ldu(g.sdl[PRIM][2], k);
             if (PIPE) {
                 commit(bn, k + 1, xn, qn, en);
                 // past the chunk: an out-of-ring plane, which loads nothing
@@ -519,17 +520,21 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
                         out = cc[u] - om2 * (ec[u] * xc[u]);
                         if (OFFD) out = out - om2 * term[u];
                     } else {
-                        // q = qs·T and the output factor lf = q·T
-                        const c64 Tu = Txy[u] * (u == 2 ? szD : szP);
-                        const c64 qu = qc[u] * Tu;
-                        c64 lf = qu * Tu;
-                        if (GD) {
-                            out = lf * cc[u] - om2 * ((ec[u] * (qu * qu)) * xc[u]);
-                        } else {
-                            c64 cd = u == 0 ? tmy + tzm : (u == 1 ? tzm + tmx : tmx + tmy);
-                            out = lf * cc[u] + (cone<double>() - cd * (qu * qu)) * xc[u];
+                        // (Q T A T⁻¹ Q x)_u with q = qs·T: the output factor
+                        // q T = qs T² =: w and the mass term −ω² ε q² x =
+                        // (1 − c q²) x give x + w (cc − c qs x); grad-div keeps
+                        // ε: w (cc − ω² ε qs x)
+                        const c64 w = qc[u] * (T2xy[u] * (u == 2 ? sdD : sdP));
+                        const c64 v = xc[u] * qc[u];
+                        c64 in;
+                        if (GD) in = cc[u] - om2 * (ec[u] * v);
+                        else {
+                            const c64 cd = u == 0 ? tmy + tzm : (u == 1 ? tzm + tmx : tmx + tmy);
+                            in = cc[u] - cd * v;
                         }
-                        if (OFFD) out = out - om2 * (lf * term[u]);
+                        if (OFFD) in = in - om2 * term[u];
+                        out = w * in;
+                        if (!GD) out = out + xc[u];
                     }
                     int64_t e = dofidx(g, u, p);
                     if (EPI == EPI_RESID) {
