@@ -2222,7 +2222,7 @@ struct BatchProb {
     int prec;                  // precision of the current cycle
     double bn = 0, worst = 1, prev = 0, best_probe = 1e300;
     int stall = 0;
-    long long it_total = 0, it32 = 0, cycles = 0, next_probe = 0;
+    long long it_total = 0, it32 = 0, cycles = 0, next_probe = 0, next64 = 0;
     bool conv = false, done = false, stalled32 = false;   // stalled32: an FP32 cycle stagnated
 };
 }  // namespace
@@ -2295,6 +2295,7 @@ int solve_batch_dev_impl(int count, mfd_op *const *ops, const void *const *b, vo
         }
         const int chunk = o.check_every > 0 ? o.check_every : 32;
         const long long probe_every = std::max<long long>(chunk, 512) / chunk * chunk;
+        const long long probe64 = std::max<long long>(chunk, 256) / chunk * chunk;
         // FP64 residual of problem p; ends it or starts its next inner cycle
         auto start_cycle = [&](BatchProb &B) -> int {
             mfd_op *op = B.op;
@@ -2339,6 +2340,7 @@ int solve_batch_dev_impl(int count, mfd_op *const *ops, const void *const *b, vo
             B.best_probe = 1e300;
             B.stall = 0;
             B.next_probe = probe_every;
+            B.next64 = 0;
             return MFD_OK;
         };
         // end of an inner cycle: e += (Q T⁻¹) z in FP64
@@ -2424,10 +2426,34 @@ int solve_batch_dev_impl(int count, mfd_op *const *ops, const void *const *b, vo
                     } else B.stall = 0;
                     B.best_probe = std::min(B.best_probe, pr);
                 }
+                if (!end && B.prec == MFD_FP64 && !B.op->gd && S.it >= B.next64 &&
+                    S.tau * B.worst <= 3.0 * o.tol * S.cn) {
+                    // FP64 cycle near the tolerance: it ends as soon as the true
+                    // residual is <= 0.9 tol (the rule of the single-problem solve)
+                    B.next64 = S.it + probe64;
+                    Workspace &w = B.op->ws;
+                    dim3 gv(ngrid(w.n), 1);
+                    k_probe_x<double><<<gv, BS, 0, s>>>(B.op->G.d64, B.op->d64, (const c64 *)B.e, (const c64 *)w.z, (c64 *)w.et, w.n, w.n);
+                    launch_apply<double, MODE_A, EPI_RESID>(B.op, (const c64 *)w.et, (c64 *)w.et2, (const c64 *)B.b, 1,
+                                                            w.part + 2 * NPART * 4, nullptr, w.rn);
+                    BCHK(hipMemcpyAsync(w.rn_h, w.rn, sizeof(double), hipMemcpyDeviceToHost, s));
+                    BCHK(hipStreamSynchronize(s));
+                    const double pr = B.bn > 0 ? sqrt(w.rn_h[0] / B.bn) : 0.0;
+                    if (getenv("MFD_SOLVE_TRACE")) fprintf(stderr, "[mfd batch probe] it %lld true residual %.3e\n", S.it, pr);
+                    end = pr <= 0.9 * o.tol;
+                }
                 if (end) {
                     end_cycle(B, S.it);
                     int r = start_cycle(B);
                     if (r) { rc = r; goto cleanup; }
+                    if (B.done) {
+                        // the captured graph may still hold this problem's
+                        // branch: its kernels skip a problem marked done
+                        const int one = 1;
+                        BCHK(hipMemcpyAsync(B.op->ws.active, &one, sizeof(int), hipMemcpyHostToDevice, s));
+                        k_set_done<<<1, 64, 0, s>>>(B.op->ws.scal, B.op->ws.active, 1);
+                        BCHK(hipStreamSynchronize(s));   // the pageable `one`
+                    }
                 }
             }
         }
