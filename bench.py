@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--nrhs", type=int, default=4, help="batched-rhs: right-hand sides")
     ap.add_argument("--count", type=int, default=32, help="uneven-batch: problems")
     ap.add_argument("--voxels", type=int, default=16, help="adjoint: finite-difference voxels")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
+                    help="batched-rhs, uneven-batch: solver precision (mixed: complex-FP32 inner cycles)")
     ap.add_argument("--maxit", type=int, default=2000, help="large-aniso: solver steps")
     return ap.parse_args()
 
@@ -126,7 +128,8 @@ def kernel_table(op, cells, iters=50):
 
 def run_batched_rhs(a):
     """N³ random boxes (the 40-box generator) with nrhs seeded dipoles solved in
-    one batched FP64 launch, against one single-RHS solve of the first dipole."""
+    one batched launch (FP64 or mixed), against one single-RHS solve of the
+    first dipole in the same precision."""
     import numpy as np
     import torch
     import maxwellfdfd as M
@@ -143,28 +146,29 @@ def run_batched_rhs(a):
     torch.cuda.synchronize()
     print(f"[batched-rhs] {N}^3, {a.nrhs} right-hand sides: batched solve", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    st = op.solve_dev(b_d.data_ptr(), e_d.data_ptr(), a.nrhs, tol=1e-8)
+    st = op.solve_dev(b_d.data_ptr(), e_d.data_ptr(), a.nrhs, tol=1e-8, precision=a.precision)
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
     print(f"[batched-rhs] batched solve {t_batch:.2f} s; single-RHS solve", file=sys.stderr, flush=True)
     e1 = torch.zeros_like(b_d[0])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    st1 = op.solve_dev(b_d[0].data_ptr(), e1.data_ptr(), 1, tol=1e-8)
+    st1 = op.solve_dev(b_d[0].data_ptr(), e1.data_ptr(), 1, tol=1e-8, precision=a.precision)
     torch.cuda.synchronize()
     t_one = time.perf_counter() - t0
     err = float(torch.linalg.vector_norm(e_d[0] - e1) / torch.linalg.vector_norm(e1))
     op.close()
-    return {"workload": f"batched-rhs {N}^3 random boxes, {a.nrhs} dipoles, FP64 SQMR to 1e-8",
-            "batched_s": round(t_batch, 3), "iterations_max": int(st.iterations), "relres_max": st.relres,
+    return {"workload": f"batched-rhs {N}^3 random boxes, {a.nrhs} dipoles, {a.precision} SQMR to 1e-8",
+            "batched_s": round(t_batch, 3), "iterations_max": int(st.iterations),
+            "iterations_fp32": int(st.iterations_fp32), "relres_max": st.relres,
             "single_rhs_s": round(t_one, 3), "single_rhs_iterations": int(st1.iterations),
             "s_per_rhs_batched": round(t_batch / a.nrhs, 3), "rel_diff_rhs0_vs_single": err}
 
 
 def run_uneven_batch(a):
     """`count` independent problems (N ~ U{48..128}, own boxes, own ω) in one
-    lockstep graph-launched batched FP64 solve, against the same problems solved
-    one after another."""
+    lockstep graph-launched batched solve (FP64 or mixed), against the same
+    problems solved one after another in the same precision."""
     import numpy as np
     import torch
     import maxwellfdfd as M
@@ -188,7 +192,8 @@ def run_uneven_batch(a):
     log("batched solve of", len(ops), "problems,", cells, "cells")
     os.environ["MFD_SOLVE_TRACE"] = "1"   # the solver reports each problem as it converges
     t0 = time.perf_counter()
-    st = ops[0].solve_batch_dev(ops[1:], [x.data_ptr() for x in bds], [x.data_ptr() for x in eds], tol=1e-8)
+    st = ops[0].solve_batch_dev(ops[1:], [x.data_ptr() for x in bds], [x.data_ptr() for x in eds], tol=1e-8,
+                                precision=a.precision)
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
     del os.environ["MFD_SOLVE_TRACE"]
@@ -197,18 +202,19 @@ def run_uneven_batch(a):
     its = []
     for op, bd, ed in zip(ops, bds, eds):
         ed.zero_()
-        its.append(int(op.solve_dev(bd.data_ptr(), ed.data_ptr(), 1, tol=1e-8).iterations))
+        its.append(int(op.solve_dev(bd.data_ptr(), ed.data_ptr(), 1, tol=1e-8, precision=a.precision).iterations))
         log("sequential problem", len(its), "iterations", its[-1])
     torch.cuda.synchronize()
     t_seq = time.perf_counter() - t0
     for op in ops:
         op.close()
     return {"workload": f"uneven-batch: {a.count} problems, N ~ U{{48..128}}, omega ~ U[2pi/30, 2pi/16], "
-                        f"FP64 SQMR to 1e-8",
+                        f"{a.precision} SQMR to 1e-8",
             "sizes": [int(m.grid.N[0]) for m, _ in probs], "cells_total": int(cells),
             "batched_s": round(t_batch, 3), "sequential_s": round(t_seq, 3),
             "speedup_batched_vs_sequential": round(t_seq / t_batch, 3),
             "iterations": [int(s.iterations) for s in st], "converged": all(bool(s.converged) for s in st),
+            "iterations_fp32": [int(s.iterations_fp32) for s in st],
             "iterations_sequential": its}
 
 
