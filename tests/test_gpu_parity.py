@@ -2,8 +2,8 @@
 
 CSC index structure (colptr/rowval) must be bit-exact; values agree to 1e-13
 relative to the matrix's largest entry; the matrix-free operator to 1e-12;
-solved fields to 1e-7 relative at FP64 residual 1e-8 (the tolerances the
-north-star contract states for values and fields).
+solved fields to 1e-7 relative at FP64 residual 1e-8 (the drop-in contract
+for operator values and solved fields).
 """
 import sys
 
