@@ -57,7 +57,7 @@ def parse():
                     help="fixed stencil tiles 'TY,KZ,order/TY,KZ,order' (FP64/FP32) instead of the autotuner, "
                          "so that separate hardware-counter passes launch the same kernels")
     ap.add_argument("--workload", default="random-boxes",
-                    choices=["random-boxes", "batched-rhs", "uneven-batch", "adjoint", "large-aniso"],
+                    choices=["random-boxes", "batched-rhs", "uneven-batch", "adjoint", "large-aniso", "next-rows"],
                     help="random-boxes: the metric line; the others time the batched-RHS, uneven-problem "
                          "and adjoint configurations")
     ap.add_argument("--nrhs", type=int, default=4, help="batched-rhs: right-hand sides")
@@ -315,6 +315,83 @@ def run_adjoint(a):
             "fd_voxels": a.voxels, "fd_s": round(t_fd, 1), "fd_rel_err_max": max(errs), "fd_rel_err": errs}
 
 
+def run_next_rows(a):
+    """The rows beyond the plain EE path, on the 64³ random-box model (FP64 to
+    1e-8): the EE solve (tiled stencil) next to the HH formulation and a
+    boundft = HH z axis (two-pass direct kernel), h_from_e / e_from_h,
+    create_Mcs, and a 1024² 2D TE solve (the single-z-cell 3D path)."""
+    import numpy as np
+    import maxwellfdfd as M
+    from maxwellfdfd import twod as T
+    from maxwellfdfd import workloads as W
+    log = lambda *x: print("[next-rows]", *x, file=sys.stderr, flush=True)
+    out = {"workload": "next rows: 64^3 random boxes (EE, HH formulation, boundft HH on z), field conversion, "
+                       "create_Mcs, 1024^2 2D TE; FP64 SQMR to 1e-8"}
+
+    def solve(op, b):
+        t0 = time.perf_counter()
+        x, st = op.solve(b, tol=1e-8)
+        return x, {"s": round(time.perf_counter() - t0, 3), "device_s": round(st.seconds, 3),
+                   "iterations": int(st.iterations), "relres": st.relres}
+
+    N = 64
+    m, om = W.random_boxes(N=N, npml=10, nbox=40, lam=24.0, seed=1, ndip=1)
+    M.calc_matparams(m)
+    je, jm = M.create_srcs(m)
+    op = M.MaxwellOperator(m, om)
+    e, out["ee_solve"] = solve(op, M.create_b(M.EE, om, m, je, jm))
+    out["ee_apply_ms"] = round(op.bench_apply(1, "fp64", iters=20), 4)
+    log("EE", out["ee_solve"])
+    t0 = time.perf_counter()
+    h_ee = op.h_from_e(e)
+    out["h_from_e_s"] = round(time.perf_counter() - t0, 4)
+    t0 = time.perf_counter()
+    e_back = op.e_from_h(h_ee, je)   # e = (−i/ω) ε⁻¹ (Cₘ h − jₑ) returns the solved e
+    out["e_from_h_s"] = round(time.perf_counter() - t0, 4)
+    out["e_from_h_of_h_from_e_rel"] = float(np.linalg.norm(e_back - e) / np.linalg.norm(e))
+    op.close()
+    oph = M.MaxwellOperator(m, om, ft=M.HH)
+    h, out["hh_solve"] = solve(oph, M.create_b(M.HH, om, m, je, jm))
+    out["hh_apply_ms"] = round(oph.bench_apply(1, "fp64", iters=20), 4)
+    out["hh_vs_h_from_e_rel"] = float(np.linalg.norm(h - h_ee) / np.linalg.norm(h_ee))
+    oph.close()
+    log("HH formulation", out["hh_solve"])
+    m2, _ = W.random_boxes(N=N, npml=10, nbox=40, lam=24.0, seed=1, ndip=0)
+    M.set_boundft(m2, (M.EE, M.EE, M.HH))   # before the deposit: it fixes the E grid types
+    for pos, v in W.random_dipoles(N, 10, 1, 1001):
+        M.add_srce(m2, M.PointSrc(pos, v, 1.0))
+    M.calc_matparams(m2)
+    je2, jm2 = M.create_srcs(m2)
+    op2 = M.MaxwellOperator(m2, om)
+    _, out["boundft_hh_z_solve"] = solve(op2, M.create_b(M.EE, om, m2, je2, jm2))
+    out["boundft_hh_z_apply_ms"] = round(op2.bench_apply(1, "fp64", iters=20), 4)
+    op2.close()
+    log("boundft HH on z", out["boundft_hh_z_solve"])
+    m128, _ = W.random_boxes(N=128, npml=10, nbox=40, lam=24.0, seed=1, ndip=0)
+    M.calc_matparams(m128)
+    t0 = time.perf_counter()
+    Mce, Mcm = M.create_Mcs(m128)
+    out["create_Mcs_128_s"] = round(time.perf_counter() - t0, 3)
+    out["create_Mcs_128_nnz"] = [int(Mce.nnz), int(Mcm.nnz)]
+    del Mce, Mcm
+    log("create_Mcs 128^3", out["create_Mcs_128_s"], "s")
+    n2 = 1024
+    lp = [np.arange(n2 + 1, dtype=np.float64)] * 2
+    m2d = T.ModelTE(T.Grid2(lp, (False, False)))
+    T.set_ωpml(m2d, om)
+    T.set_Npml(m2d, ((10, 10), (10, 10)))
+    T.add_obj(m2d, "bg", T.Rect((n2 / 2, n2 / 2), (float(n2), float(n2))), ε=1.0)
+    T.add_obj(m2d, "rod", T.Disk((n2 / 2 + 40.0, n2 / 2 - 25.0), 120.0), ε=9.0)
+    T.add_srce(m2d, T.PointSrc2((n2 / 2 - 200.5, n2 / 2 + 0.5), (0.0, 1.0)))
+    T.calc_matparams(m2d)
+    je2d, jm2d = T.create_srcs(m2d)
+    op2d = T.MaxwellOperator2D(m2d, om)
+    _, out["te2d_1024_solve"] = solve(op2d, T.create_b(M.EE, om, m2d, je2d, jm2d))
+    op2d.close()
+    log("2D TE 1024^2", out["te2d_1024_solve"])
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -330,6 +407,8 @@ def main():
             out = run_uneven_batch(a)
         elif a.workload == "large-aniso":
             out = run_large_aniso(a)
+        elif a.workload == "next-rows":
+            out = run_next_rows(a)
         else:
             out = run_adjoint(a)
         print(json.dumps(out))
