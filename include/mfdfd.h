@@ -181,7 +181,9 @@ int mfd_op_set_graddiv(mfd_op *op, int enable);
  * per precision.  Outputs {TY, KZ, order} may be null.  No reference counterpart. */
 int mfd_op_autotune(mfd_op *op, int64_t nrhs, int iters, int tile64[3], int tile32[3], double ms[2]);
 /* Fix the tile shape by hand: TY in {1,2,4,8} (0 = cache-reliant direct
- * kernel), KZ >= 1, xcd_order 0 or 1. */
+ * kernel), KZ >= 1, xcd_order 0 or 1.  Until this or mfd_op_autotune is
+ * called, an operator uses TY = 4, KZ = 16, xcd_order 1, with KZ and then TY
+ * halved (down to 4 and 1) while a launch would have fewer than 256 tiles. */
 int mfd_op_set_tile(mfd_op *op, int prec, int ty, int kz, int xcd_order);
 /* Average device time (ms) of one apply of the solver's stencil kernel
  * (MFD_FP64: A; MFD_MIXED: the complex-FP32 inner operator) */

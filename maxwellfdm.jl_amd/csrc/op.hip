@@ -1140,6 +1140,20 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
         if (op->G.d64.bloch[a] && fabs(op->G.d64.ph[a].im) > 1e-14) op->sym = 0;
     op->gd_ok = !full && !op->has_mu && op->ee_all && !(om2.re == 0.0 && om2.im == 0.0);
     if (!op->ee_all) op->tile64[0] = op->tile32[0] = 0;   // the direct kernel serves other boundaries
+    else {
+        // default tile (until mfd_op_autotune or mfd_op_set_tile): 4 rows x 16
+        // planes, with fewer planes and then fewer rows per block on grids too
+        // small for that to give one tile per CU (256)
+        const int64_t nx = (op->G.h.N[0] + 63) / 64;
+        auto tiles = [&](int ty, int kz) {
+            return nx * ((op->G.h.N[1] + ty - 1) / ty) * ((op->G.h.N[2] + kz - 1) / kz);
+        };
+        int ty = 4, kz = 16;
+        while (kz > 4 && tiles(ty, kz) < 256) kz /= 2;
+        while (ty > 1 && tiles(ty, kz) < 256) ty /= 2;
+        op->tile64[0] = op->tile32[0] = ty;
+        op->tile64[1] = op->tile32[1] = kz;
+    }
     op->gd = 0;   // opt-in: it helps small PML-dominated grids and slows larger ones
     k_wnode<<<nb(M), BS, 0, s>>>(op->G.d64, epsd, wn, wn32);
     if (op->gd) k_jacobi_gd<<<nb(M), BS, 0, s>>>(op->G.d64, op->d64, q);
