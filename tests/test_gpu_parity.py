@@ -267,6 +267,61 @@ def test_apply_matches_csc(case, full, mu, pkg, O):
     op.close()
 
 
+@pytest.mark.parametrize("seed", range(24))
+def test_apply_random_configs(seed, pkg, O):
+    """Seeded random configurations against the oracle's CSC A: grid sizes
+    1–21 (one axis sometimes 60–79, past a 64-wide tile), uniform or
+    nonuniform spacing, Bloch axes with real phases, PML on random ends,
+    boundft and ft per case, μ ≠ 1, full ε where allowed, and a random tile
+    shape (TY = 0: the direct kernel)."""
+    rng = np.random.default_rng(1000 + seed)
+    N = tuple(int(v) for v in rng.integers(1, 22, 3))
+    if rng.random() < 0.2:
+        N = (int(rng.integers(60, 80)),) + N[1:]
+    bloch = tuple(bool(N[a] == 1 or rng.random() < 0.4) for a in range(3))
+    npml = [[0, 0, 0], [0, 0, 0]]
+    for a in range(3):
+        for e in range(2):
+            if N[a] >= 6 and rng.random() < 0.6:
+                npml[e][a] = int(rng.integers(1, 3))
+    kb = tuple(float(rng.uniform(-0.4, 0.4)) if bloch[a] and rng.random() < 0.5 else 0.0 for a in range(3))
+    bft = tuple(int(rng.random() < 0.3) for a in range(3))
+    ft = int(rng.random() < 0.3)
+    lp = ([np.cumsum(np.concatenate(([0.0], rng.uniform(0.6, 1.4, n)))) for n in N] if rng.random() < 0.5
+          else [np.arange(n + 1.0) for n in N])
+    m = pkg.ModelFull(pkg.Grid(lp, bloch))
+    pkg.set_ωpml(m, 0.7)
+    pkg.set_Npml(m, (tuple(npml[0]), tuple(npml[1])))
+    pkg.set_kbloch(m, kb)
+    pkg.set_boundft(m, bft)
+    mu_on = rng.random() < 0.3
+    full = ft == 0 and bft == (0, 0, 0) and rng.random() < 0.3
+    eps = np.zeros(N + (3, 3), dtype=np.complex128, order="F")
+    muu = np.zeros_like(eps)
+    for c in range(3):
+        eps[..., c, c] = 1 + 11 * rng.random(N) + 0.1j * rng.random(N)
+        muu[..., c, c] = (1 + 0.5 * rng.random(N)) if mu_on else 1.0
+    if full:
+        for (v, w) in ((0, 1), (0, 2), (1, 2)):
+            x = rng.normal(size=N) + 0.1j * rng.normal(size=N)
+            x[rng.random(N) < 0.5] = 0
+            eps[..., v, w] = x
+            eps[..., w, v] = x
+    om = 0.55 + 0.03j * rng.random()
+    ty, kz, xo = int(rng.choice([0, 1, 2, 4, 8])), int(rng.integers(1, 17)), int(rng.integers(0, 2))
+    say("random config", seed, N, bloch, npml, bft, ft, mu_on, full, (ty, kz, xo))
+    A = O.to_scipy(O.create_A(spec_of(m), om, eps, muu, ft=ft))
+    op = pkg.MaxwellOperator(m, om, ft=pkg.HH if ft else pkg.EE, εarr=eps, μarr=muu)
+    op.set_tile("fp64", ty, kz, xo)
+    n = 3 * m.grid.M
+    X = rng.normal(size=(2, n)) + 1j * rng.normal(size=(2, n))
+    Y = op.apply(X)
+    op.close()
+    for r in range(2):
+        ref = A @ X[r]
+        assert np.linalg.norm(Y[r] - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
 HHCASES = {
     "boundft_HH": CASES["boundft_HH"],
     "hh_pml": dict(N=(5, 6, 4), npml=((1, 1, 0), (1, 0, 1)), om=0.7, bloch=(False,) * 3, bft=(1, 0, 1), nu=3),
