@@ -197,7 +197,9 @@ int mfd_op_bench_iteration(mfd_op *op, int64_t nrhs, int prec, int iters, double
 /* Solve A e = b (the user's `A \ b`); e holds the initial guess on entry.
  * The result meets tol on the FP64 residual b − A e (computed with the FP64
  * operator after each cycle); SQMR FP64 cycles end as soon as a probe of that
- * residual is at or below 0.9·tol. */
+ * residual is at or below 0.9·tol.  A SQMR or BiCGSTAB solve that ends
+ * without converging (MFD_ERR_NOCONV) leaves in e the iterate with the
+ * smallest such residual, which stats.relres reports. */
 void mfd_solve_opts_default(mfd_solve_opts *o);
 int mfd_solve(mfd_op *op, const mfd_c64 *b, mfd_c64 *e, int64_t nrhs, const mfd_solve_opts *o,
               mfd_solve_stats *st);

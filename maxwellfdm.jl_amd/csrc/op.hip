@@ -895,7 +895,7 @@ static int ws_ensure(mfd_op *op, int64_t nrhs, int prec) {
     size_t vb = sizeof(c64) * n * nrhs;
     size_t align = 256;
     auto up = [&](size_t x) { return (x + align - 1) / align * align; };
-    size_t bytes = 5 * up(vb) + 3 * up(sizeof(c64) * n * nrhs) + up(sizeof(c64) * (n / 3) * nrhs) +
+    size_t bytes = 5 * up(vb) + 4 * up(sizeof(c64) * n * nrhs) + up(sizeof(c64) * (n / 3) * nrhs) +
                    up(sizeof(double) * 3 * nrhs * NPART * 4) +
                    up(sizeof(unsigned) * 3 * nrhs) + up(sizeof(Scal) * nrhs) + 2 * up(sizeof(double) * nrhs) +
                    up(sizeof(int) * 3 * nrhs);
@@ -909,6 +909,7 @@ static int ws_ensure(mfd_op *op, int64_t nrhs, int prec) {
     w.r64 = p; p += up(sizeof(c64) * n * nrhs);
     w.et = p; p += up(sizeof(c64) * n * nrhs);
     w.et2 = p; p += up(sizeof(c64) * n * nrhs);
+    w.ebest = p; p += up(sizeof(c64) * n * nrhs);
     w.psi = p; p += up(sizeof(c64) * (n / 3) * nrhs);
     w.part = (double *)p; p += up(sizeof(double) * 3 * nrhs * NPART * 4);
     w.cnt = (unsigned *)p; p += up(sizeof(unsigned) * 3 * nrhs);
@@ -2143,7 +2144,7 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
     std::vector<int> active(nrhs, 1);
     int *active_d = w.active;
     long long total_it = 0, cycles = 0, it32 = 0;
-    double worst = 0, prev_worst = 0;
+    double worst = 0, prev_worst = 0, best = 1e300;
     bool conv = false;
     // FP32 cycles run while each gains at least 10x on the FP64 residual and
     // ends at its target; a cycle that gains less, or whose probes stopped
@@ -2166,7 +2167,14 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
             if (rel > worst) worst = rel;
         }
         if (all) { conv = true; break; }
-        if (total_it >= o.maxit) break;
+        // the iterate with the smallest residual at a cycle start is kept: a
+        // solve that ends without converging returns it (a cycle of a
+        // stagnating recurrence can raise the residual)
+        if (worst < best) {
+            best = worst;
+            MFD_HIP(hipMemcpyAsync(w.ebest, e, sizeof(c64) * n * nrhs, hipMemcpyDeviceToDevice, s));
+        }
+        if (total_it >= o.maxit || cycles > 200) break;
         if (cyc_prec == MFD_MIXED && cycles > 0 && (worst > 0.1 * prev_worst || stalled32)) cyc_prec = MFD_FP64;
         prev_worst = worst;
         MFD_HIP(hipMemcpyAsync(active_d, active.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s));
@@ -2204,7 +2212,10 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
         if (cyc_prec == MFD_MIXED) k_upd_e<float><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (c64 *)e, (const c32 *)w.z, n, n);
         else k_upd_e<double><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (c64 *)e, (const c64 *)w.z, n, n);
         MFD_HIP(hipGetLastError());
-        if (cycles > 200) break;
+    }
+    if (!conv && best < worst) {
+        MFD_HIP(hipMemcpyAsync(e, w.ebest, sizeof(c64) * n * nrhs, hipMemcpyDeviceToDevice, s));
+        worst = best;
     }
     hipEventRecord(op->ev1, s);
     hipEventSynchronize(op->ev1);

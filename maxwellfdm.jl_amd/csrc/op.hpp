@@ -41,6 +41,7 @@ struct Workspace {
     void *r64 = nullptr;      // outer FP64 residual
     void *psi = nullptr;      // FP64 node scratch for the grad-div residual transform
     void *et = nullptr, *et2 = nullptr;   // FP64 scratch for true-residual probes
+    void *ebest = nullptr;    // FP64 iterate with the smallest residual at a cycle start
     double *part = nullptr;   // reduction partials [slot][rhs][NPART][4]
     unsigned *cnt = nullptr;  // [slot][rhs]
     Scal *scal = nullptr;     // [rhs]

@@ -708,6 +708,22 @@ def test_solve_iteration_cap_reports(pkg):
     op.close()
 
 
+@pytest.mark.parametrize("method", ["sqmr", "bicgstab"])
+def test_noconv_returns_best_iterate(method, pkg, O):
+    """A solve that ends without converging leaves the iterate with the
+    smallest FP64 residual at a cycle start (never worse than the initial
+    guess) and reports that iterate's residual."""
+    m, om, b = c1_like(pkg, N=10, npml=2, r=2.0, lam=7.0)
+    A = O.to_scipy(O.create_A(spec_of(m), om, m.εarr, m.μarr))
+    op = pkg.MaxwellOperator(m, om)
+    say("noconv best iterate", method)
+    e, st = op.solve(b, tol=1e-15, maxit=600, check_every=8, method=method, raise_on_noconv=False)
+    op.close()
+    rel = np.linalg.norm(b - A @ e) / np.linalg.norm(b)
+    assert not st.converged and st.relres <= 1.0
+    assert abs(rel - st.relres) <= 1e-6 * st.relres + 1e-14
+
+
 def test_adjoint(pkg, O):
     """Aᵀλ = g against a direct solve of the transpose; ∂f/∂ε against central
     finite differences (h = 1e-4) for f = Σ|e_z|² over a monitor."""
