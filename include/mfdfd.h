@@ -197,9 +197,13 @@ int mfd_op_bench_iteration(mfd_op *op, int64_t nrhs, int prec, int iters, double
 /* Solve A e = b (the user's `A \ b`); e holds the initial guess on entry.
  * The result meets tol on the FP64 residual b − A e (computed with the FP64
  * operator after each cycle); SQMR FP64 cycles end as soon as a probe of that
- * residual is at or below 0.9·tol.  A SQMR or BiCGSTAB solve that ends
- * without converging (MFD_ERR_NOCONV) leaves in e the iterate with the
- * smallest such residual, which stats.relres reports. */
+ * residual is at or below 0.9·tol.  A right-hand side b = 0 gets e = 0 (the
+ * exact solution) whatever the initial guess.  A right-hand side whose
+ * residual is NaN stops iterating and the solve returns MFD_ERR_NOCONV with
+ * stats.relres NaN.  A SQMR or BiCGSTAB solve that ends without converging
+ * (MFD_ERR_NOCONV) leaves in each column of e the iterate with that column's
+ * smallest such residual (converged columns keep their converged iterate);
+ * stats.relres reports the largest of those residuals. */
 void mfd_solve_opts_default(mfd_solve_opts *o);
 int mfd_solve(mfd_op *op, const mfd_c64 *b, mfd_c64 *e, int64_t nrhs, const mfd_solve_opts *o,
               mfd_solve_stats *st);
@@ -211,6 +215,8 @@ int mfd_solve_dev(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_so
  * Each chunk of iterations of all unconverged problems is one hipGraph launch
  * whose per-problem branches run concurrently.  b[p], e[p]: host arrays of
  * 3·M_p values (e holds the initial guess); st: `count` entries or null.
+ * b = 0, NaN residuals and unconverged problems are handled per problem as
+ * mfd_solve handles its columns.
  * SQMR only.  No reference counterpart (the reference solves one `A \ b`). */
 int mfd_solve_batch(int count, mfd_op *const *ops, const mfd_c64 *const *b, mfd_c64 *const *e,
                     const mfd_solve_opts *o, mfd_solve_stats *st);

@@ -851,6 +851,10 @@ static inline unsigned ngrid(int64_t n) {
 }
 
 void Workspace::release() {
+    if (fixed) {   // caller-provided memory: forget it, free nothing
+        *this = Workspace();
+        return;
+    }
     if (graph) hipGraphExecDestroy(graph);
     graph = nullptr;
     if (buf) hipFree(buf);
@@ -884,47 +888,63 @@ static int hbuf_ensure(mfd_op *op, int64_t nrhs) {
     return MFD_OK;
 }
 
+static inline size_t ws_up(size_t x) { return (x + 255) / 256 * 256; }
+
+// Bytes of a solver workspace for nrhs right-hand sides of n unknowns.  The
+// Krylov vectors are sized for FP64 in both modes: a mixed solve whose FP32
+// cycles stop paying off continues with FP64 cycles in place.
+size_t ws_bytes(int64_t n, int64_t nrhs) {
+    const size_t vb = ws_up(sizeof(c64) * n * nrhs);
+    return 9 * vb + ws_up(sizeof(c64) * (n / 3) * nrhs) + ws_up(sizeof(double) * 3 * nrhs * NPART * 4) +
+           ws_up(sizeof(unsigned) * 3 * nrhs) + ws_up(sizeof(Scal) * nrhs) + 2 * ws_up(sizeof(double) * nrhs) +
+           ws_up(sizeof(int) * 3 * nrhs);
+}
+
+// Lays out the device part of a workspace of ws_bytes(n, nrhs) bytes at base
+// (256-byte aligned); the host mirrors scal_h and rn_h are the caller's.
+void ws_carve(Workspace &w, void *base, int64_t n, int64_t nrhs) {
+    const size_t vb = ws_up(sizeof(c64) * n * nrhs);
+    char *p = (char *)base;
+    w.r = p; p += vb;
+    w.q = p; p += vb;
+    w.t = p; p += vb;
+    w.d = p; p += vb;
+    w.z = p; p += vb;
+    w.r64 = p; p += vb;
+    w.et = p; p += vb;
+    w.et2 = p; p += vb;
+    w.ebest = p; p += vb;
+    w.psi = p; p += ws_up(sizeof(c64) * (n / 3) * nrhs);
+    w.part = (double *)p; p += ws_up(sizeof(double) * 3 * nrhs * NPART * 4);
+    w.cnt = (unsigned *)p; p += ws_up(sizeof(unsigned) * 3 * nrhs);
+    w.scal = (Scal *)p; p += ws_up(sizeof(Scal) * nrhs);
+    w.bn = (double *)p; p += ws_up(sizeof(double) * nrhs);
+    w.rn = (double *)p; p += ws_up(sizeof(double) * nrhs);
+    w.active = (int *)p;
+    w.nrhs = nrhs;
+    w.n = n;
+    w.bytes = ws_bytes(n, nrhs);
+    w.graph_iters = 0;
+}
+
 static int ws_ensure(mfd_op *op, int64_t nrhs, int prec) {
     Workspace &w = op->ws;
     int64_t n = 3 * op->G.h.M;
+    if (w.fixed) {   // a caller-provided workspace: never reallocated here
+        if (w.nrhs != nrhs || w.n != n) { set_error("fixed workspace does not match nrhs"); return MFD_ERR_ARG; }
+        w.prec = prec;
+        return MFD_OK;
+    }
     MFD_TRY(hbuf_ensure(op, nrhs));
     if (w.buf && w.nrhs == nrhs && w.prec == prec) return MFD_OK;
     w.release();
-    // Krylov vectors are sized for FP64 in both modes: a mixed solve whose
-    // FP32 cycles stop paying off continues with FP64 cycles in place
-    size_t vb = sizeof(c64) * n * nrhs;
-    size_t align = 256;
-    auto up = [&](size_t x) { return (x + align - 1) / align * align; };
-    size_t bytes = 5 * up(vb) + 4 * up(sizeof(c64) * n * nrhs) + up(sizeof(c64) * (n / 3) * nrhs) +
-                   up(sizeof(double) * 3 * nrhs * NPART * 4) +
-                   up(sizeof(unsigned) * 3 * nrhs) + up(sizeof(Scal) * nrhs) + 2 * up(sizeof(double) * nrhs) +
-                   up(sizeof(int) * 3 * nrhs);
+    const size_t bytes = ws_bytes(n, nrhs);
     MFD_HIP(hipMalloc(&w.buf, bytes));
-    char *p = (char *)w.buf;
-    w.r = p; p += up(vb);
-    w.q = p; p += up(vb);
-    w.t = p; p += up(vb);
-    w.d = p; p += up(vb);
-    w.z = p; p += up(vb);
-    w.r64 = p; p += up(sizeof(c64) * n * nrhs);
-    w.et = p; p += up(sizeof(c64) * n * nrhs);
-    w.et2 = p; p += up(sizeof(c64) * n * nrhs);
-    w.ebest = p; p += up(sizeof(c64) * n * nrhs);
-    w.psi = p; p += up(sizeof(c64) * (n / 3) * nrhs);
-    w.part = (double *)p; p += up(sizeof(double) * 3 * nrhs * NPART * 4);
-    w.cnt = (unsigned *)p; p += up(sizeof(unsigned) * 3 * nrhs);
-    w.scal = (Scal *)p; p += up(sizeof(Scal) * nrhs);
-    w.bn = (double *)p; p += up(sizeof(double) * nrhs);
-    w.rn = (double *)p; p += up(sizeof(double) * nrhs);
-    w.active = (int *)p; p += up(sizeof(int) * 3 * nrhs);
+    ws_carve(w, w.buf, n, nrhs);
     MFD_HIP(hipMemsetAsync(w.cnt, 0, sizeof(unsigned) * 3 * nrhs, op->stream));
     MFD_HIP(hipHostMalloc((void **)&w.scal_h, sizeof(Scal) * nrhs, hipHostMallocDefault));
     MFD_HIP(hipHostMalloc((void **)&w.rn_h, sizeof(double) * nrhs, hipHostMallocDefault));
-    w.nrhs = nrhs;
-    w.n = n;
     w.prec = prec;
-    w.bytes = bytes;
-    w.graph_iters = 0;
     return MFD_OK;
 }
 
@@ -1573,6 +1593,37 @@ static void launch_cocg_iter(mfd_op *op, int64_t nrhs, int par) {
     k_fin_cocg<<<(unsigned)nrhs, BS, 0, s>>>(w.part + pslot, (int)gv.x, w.scal);
 }
 
+// ‖b_r‖² of every RHS into bn (host); a zero right-hand side has the exact
+// solution 0, which is stored into its column of e here (whatever the initial
+// guess), so every solve path then treats that RHS as converged
+static int rhs_norms(mfd_op *op, const void *b, void *e, int64_t nrhs, double *bn) {
+    Workspace &w = op->ws;
+    hipStream_t s = op->stream;
+    const int64_t n = w.n;
+    const size_t pslot = (size_t)nrhs * NPART * 4;
+    k_norm2<<<dim3(ngrid(n), (unsigned)nrhs), BS, 0, s>>>((const c64 *)b, n, n, w.part + 2 * pslot, w.cnt + 2 * nrhs, w.bn);
+    MFD_HIP(hipGetLastError());
+    MFD_HIP(hipMemcpyAsync(bn, w.bn, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+    MFD_HIP(hipStreamSynchronize(s));
+    for (int64_t r = 0; r < nrhs; ++r)
+        if (bn[r] == 0.0) MFD_HIP(hipMemsetAsync((c64 *)e + r * n, 0, sizeof(c64) * n, s));
+    return MFD_OK;
+}
+
+// relative residual from ‖r‖² and ‖b‖² (0 for a zero right-hand side, whose
+// column rhs_norms set to the exact solution; NaN stays NaN)
+static inline double rel_residual(double rn2, double bn2) { return bn2 == 0.0 ? 0.0 : sqrt(rn2 / bn2); }
+
+// largest relative residual, NaN if any is NaN
+static inline double max_rel(const std::vector<double> &rel) {
+    double m = 0.0;
+    for (double v : rel) {
+        if (std::isnan(v)) return v;
+        m = std::max(m, v);
+    }
+    return m;
+}
+
 // COCG with reliable updates; b, e device FP64; returns iterations and updates
 template <typename R>
 static int run_cocg(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_solve_opts &o, double delta,
@@ -1586,14 +1637,13 @@ static int run_cocg(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_
     const DGrid<R> &g = op->G.view<R>();
     const OpData<R> &od = opd<R>(op);
     std::vector<double> bn(nrhs);
-    k_norm2<<<gv, BS, 0, s>>>((const c64 *)b, n, n, w.part + 2 * pslot, w.cnt + 2 * nrhs, w.bn);
-    MFD_HIP(hipMemcpyAsync(bn.data(), w.bn, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+    MFD_TRY(rhs_norms(op, b, e, nrhs, bn.data()));
     k_scal_reset<<<(unsigned)((nrhs + 63) / 64), 64, 0, s>>>(w.scal, nrhs);
     MFD_HIP(hipMemsetAsync(w.z, 0, sizeof(cx<R>) * n * nrhs, s));
     MFD_HIP(hipMemsetAsync(w.q, 0, sizeof(cx<R>) * n * nrhs, s));
     MFD_HIP(hipMemsetAsync(w.d, 0, sizeof(cx<R>) * n * nrhs, s));
     int par = 0;   // parity of the next step (which p buffer holds the current p)
-    std::vector<int> mask(nrhs, 1), restart(nrhs, 1), done(nrhs, 0);
+    std::vector<int> mask(nrhs, 1), restart(nrhs, 1), done(nrhs, 0), failed(nrhs, 0);
     std::vector<double> relres(nrhs, 1.0);
     std::vector<long long> last_ru(nrhs, 0);
     int *mask_d = w.active, *aux_d = w.active + nrhs;   // mask and restart/done flags on the device
@@ -1609,15 +1659,18 @@ static int run_cocg(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_
         MFD_HIP(hipGetLastError());
         MFD_HIP(hipMemcpyAsync(w.rn_h, w.rn, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
         MFD_HIP(hipStreamSynchronize(s));
-        *worst = 0;
+        std::vector<double> open_rel;
         for (int64_t r = 0; r < nrhs; ++r) {
             if (mask[r]) {
-                double rel = bn[r] > 0 ? sqrt(w.rn_h[r] / bn[r]) : 0.0;
+                double rel = rel_residual(w.rn_h[r], bn[r]);
                 relres[r] = rel;
                 if (rel <= o.tol) done[r] = 1;
+                // a NaN residual stops that RHS without converging it
+                if (std::isnan(rel)) done[r] = failed[r] = 1;
             }
-            if (!done[r] && relres[r] > *worst) *worst = relres[r];
+            if (!done[r] || failed[r]) open_rel.push_back(relres[r]);
         }
+        *worst = max_rel(open_rel);
         return MFD_OK;
     };
     std::vector<int> m2(nrhs);
@@ -1709,12 +1762,13 @@ static int run_cocg(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_
             MFD_TRY(residual(&worst));
         }
     }
-    worst = 0;
-    for (int64_t r = 0; r < nrhs; ++r) worst = std::max(worst, relres[r]);
+    worst = max_rel(relres);
+    bool nan_rhs = false;
+    for (int64_t r = 0; r < nrhs; ++r) nan_rhs = nan_rhs || failed[r];
     *iters_out = mx;
     *updates_out = updates;
     *worst_out = worst;
-    *conv_out = all;
+    *conv_out = all && !nan_rhs;
     return MFD_OK;
 }
 
@@ -2138,13 +2192,16 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
         if (!conv) { set_error("solve did not reach the tolerance within maxit"); return MFD_ERR_NOCONV; }
         return MFD_OK;
     }
-    k_norm2<<<gv, BS, 0, s>>>((const c64 *)b, n, n, w.part + 2 * pslot, w.cnt + 2 * nrhs, w.bn);
     std::vector<double> bn(nrhs);
-    MFD_HIP(hipMemcpyAsync(bn.data(), w.bn, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+    MFD_TRY(rhs_norms(op, b, e, nrhs, bn.data()));
     std::vector<int> active(nrhs, 1);
     int *active_d = w.active;
     long long total_it = 0, cycles = 0, it32 = 0;
-    double worst = 0, prev_worst = 0, best = 1e300;
+    double worst = 0, prev_worst = 0;
+    // per RHS: the smallest FP64 residual seen at a cycle start (its iterate
+    // is kept in ebest), the latest one, and whether it is NaN (that RHS stops)
+    std::vector<double> best(nrhs, HUGE_VAL), rel(nrhs, HUGE_VAL);
+    std::vector<char> failed(nrhs, 0);
     bool conv = false;
     // FP32 cycles run while each gains at least 10x on the FP64 residual and
     // ends at its target; a cycle that gains less, or whose probes stopped
@@ -2158,23 +2215,27 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
         MFD_HIP(hipGetLastError());
         MFD_HIP(hipMemcpyAsync(w.rn_h, w.rn, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
         MFD_HIP(hipStreamSynchronize(s));
-        worst = 0;
-        bool all = true;
+        bool all = true, any_active = false;
         for (int64_t r = 0; r < nrhs; ++r) {
-            double rel = bn[r] > 0 ? sqrt(w.rn_h[r] / bn[r]) : 0.0;
-            active[r] = rel > o.tol;
-            if (active[r]) all = false;
-            if (rel > worst) worst = rel;
+            rel[r] = rel_residual(w.rn_h[r], bn[r]);
+            // a NaN residual is neither converged nor worth iterating on
+            if (std::isnan(rel[r])) failed[r] = 1;
+            const bool ok = rel[r] <= o.tol;
+            active[r] = !ok && !failed[r];
+            all = all && ok;
+            any_active = any_active || active[r];
+            // the iterate with the smallest residual at a cycle start is kept
+            // per RHS: a solve that ends without converging returns it (a
+            // cycle of a stagnating recurrence can raise the residual)
+            if (rel[r] < best[r]) {
+                best[r] = rel[r];
+                MFD_HIP(hipMemcpyAsync((c64 *)w.ebest + r * n, (const c64 *)e + r * n, sizeof(c64) * n,
+                                       hipMemcpyDeviceToDevice, s));
+            }
         }
+        worst = max_rel(rel);
         if (all) { conv = true; break; }
-        // the iterate with the smallest residual at a cycle start is kept: a
-        // solve that ends without converging returns it (a cycle of a
-        // stagnating recurrence can raise the residual)
-        if (worst < best) {
-            best = worst;
-            MFD_HIP(hipMemcpyAsync(w.ebest, e, sizeof(c64) * n * nrhs, hipMemcpyDeviceToDevice, s));
-        }
-        if (total_it >= o.maxit || cycles > 200) break;
+        if (!any_active || total_it >= o.maxit || cycles > 200) break;
         if (cyc_prec == MFD_MIXED && cycles > 0 && (worst > 0.1 * prev_worst || stalled32)) cyc_prec = MFD_FP64;
         prev_worst = worst;
         MFD_HIP(hipMemcpyAsync(active_d, active.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s));
@@ -2213,9 +2274,17 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
         else k_upd_e<double><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (c64 *)e, (const c64 *)w.z, n, n);
         MFD_HIP(hipGetLastError());
     }
-    if (!conv && best < worst) {
-        MFD_HIP(hipMemcpyAsync(e, w.ebest, sizeof(c64) * n * nrhs, hipMemcpyDeviceToDevice, s));
-        worst = best;
+    if (!conv) {
+        // columns whose last residual is above their best (or NaN) go back to
+        // the best iterate; converged columns keep theirs
+        for (int64_t r = 0; r < nrhs; ++r) {
+            if (!(rel[r] > best[r]) && !std::isnan(rel[r])) continue;
+            if (best[r] == HUGE_VAL) continue;   // never finite: nothing to restore
+            MFD_HIP(hipMemcpyAsync((c64 *)e + r * n, (const c64 *)w.ebest + r * n, sizeof(c64) * n,
+                                   hipMemcpyDeviceToDevice, s));
+            rel[r] = best[r];
+        }
+        worst = max_rel(rel);
     }
     hipEventRecord(op->ev1, s);
     hipEventSynchronize(op->ev1);
@@ -2246,6 +2315,7 @@ struct BatchProb {
     void *e;
     int prec;                  // precision of the current cycle
     double bn = 0, worst = 1, prev = 0, best_probe = 1e300;
+    double best = HUGE_VAL;    // smallest FP64 residual at a cycle start (iterate kept in ebest)
     int stall = 0;
     long long it_total = 0, it32 = 0, cycles = 0, next_probe = 0, next64 = 0;
     bool conv = false, done = false, stalled32 = false;   // stalled32: an FP32 cycle stagnated
@@ -2317,6 +2387,8 @@ int solve_batch_dev_impl(int count, mfd_op *const *ops, const void *const *b, vo
             BCHK(hipMemcpyAsync(ops[p]->ws.rn_h, ops[p]->ws.bn, sizeof(double), hipMemcpyDeviceToHost, s));
             BCHK(hipStreamSynchronize(s));
             P[p].bn = ops[p]->ws.rn_h[0];
+            // b = 0: the exact solution is 0 whatever the initial guess
+            if (P[p].bn == 0.0) BCHK(hipMemsetAsync(e[p], 0, sizeof(c64) * ops[p]->ws.n, s));
         }
         const int chunk = o.check_every > 0 ? o.check_every : 32;
         const long long probe_every = std::max<long long>(chunk, 512) / chunk * chunk;
@@ -2331,7 +2403,7 @@ int solve_batch_dev_impl(int count, mfd_op *const *ops, const void *const *b, vo
                                                     w.part + 2 * NPART * 4, nullptr, w.rn);
             MFD_HIP(hipMemcpyAsync(w.rn_h, w.rn, sizeof(double), hipMemcpyDeviceToHost, s));
             MFD_HIP(hipStreamSynchronize(s));
-            double rel = B.bn > 0 ? sqrt(w.rn_h[0] / B.bn) : 0.0;
+            const double rel = rel_residual(w.rn_h[0], B.bn);
             B.worst = rel;
             if (rel <= o.tol) {
                 B.conv = B.done = true;
@@ -2340,7 +2412,18 @@ int solve_batch_dev_impl(int count, mfd_op *const *ops, const void *const *b, vo
                             (long long)op->G.h.M, B.it_total);
                 return MFD_OK;
             }
-            if (B.it_total >= o.maxit || B.cycles > 200) { B.done = true; return MFD_OK; }
+            if (rel < B.best) {   // the best iterate at a cycle start, returned if the problem does not converge
+                B.best = rel;
+                MFD_HIP(hipMemcpyAsync(w.ebest, B.e, sizeof(c64) * n, hipMemcpyDeviceToDevice, s));
+            }
+            if (std::isnan(rel) || B.it_total >= o.maxit || B.cycles > 200) {
+                B.done = true;
+                if (B.best < HUGE_VAL && !(rel <= B.best)) {
+                    MFD_HIP(hipMemcpyAsync(B.e, w.ebest, sizeof(c64) * n, hipMemcpyDeviceToDevice, s));
+                    B.worst = B.best;
+                }
+                return MFD_OK;
+            }
             if (B.prec == MFD_MIXED && B.cycles > 0 && (rel > 0.1 * B.prev || B.stalled32)) B.prec = MFD_FP64;
             B.prev = rel;
             if (op->gd) {

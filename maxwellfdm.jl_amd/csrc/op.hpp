@@ -53,8 +53,14 @@ struct Workspace {
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0;
     int graph_kind = 0;       // 1/3 = SQMR FP32/FP64, 2 = COCG, 4 = BiCGSTAB iteration
+    bool fixed = false;       // memory the solver must not free or reallocate (caller-provided)
     void release();
 };
+
+// Solver workspace layout (op.hip): its size for nrhs right-hand sides of n
+// unknowns, and the carving of its device part from a 256-byte aligned base.
+size_t ws_bytes(int64_t n, int64_t nrhs);
+void ws_carve(Workspace &w, void *base, int64_t n, int64_t nrhs);
 
 }  // namespace mfd
 

@@ -495,6 +495,16 @@ class MaxwellOperator:
         check(lib().mfd_op_create(context(), C.byref(g), ft, L.cplx(ω), ptr(e), ptr(u), C.byref(h)))
         self._h = h
 
+    def _nrhs(self, v, what):
+        """Number of stacked DOF vectors in v; raises unless v holds a positive whole number of them."""
+        if v.size == 0 or v.size % self.n:
+            raise MfdArgumentError(L.MFD_ERR_ARG, f"{what} must hold k·{self.n} values (k ≥ 1), got {v.size}")
+        return v.size // self.n
+
+    def _one(self, v, what):
+        if v.size != self.n:
+            raise MfdArgumentError(L.MFD_ERR_ARG, f"{what} must hold {self.n} values, got {v.size}")
+
     def close(self):
         if getattr(self, "_h", None):
             lib().mfd_op_destroy(self._h)
@@ -508,7 +518,7 @@ class MaxwellOperator:
 
     def apply(self, x):
         x = L.carr(x)
-        nrhs = x.size // self.n
+        nrhs = self._nrhs(x, "x")
         y = np.zeros_like(x)
         check(lib().mfd_op_apply(self._h, ptr(x), ptr(y), nrhs))
         return y
@@ -570,8 +580,10 @@ class MaxwellOperator:
     def solve(self, b, tol=1e-8, precision="fp64", x0=None, maxit=None, inner_tol=None,
               check_every=None, use_graph=True, raise_on_noconv=True, method=None):
         b = L.carr(b)
-        nrhs = b.size // self.n
+        nrhs = self._nrhs(b, "b")
         e = np.zeros_like(b) if x0 is None else L.carr(x0).copy()
+        if e.size != b.size:
+            raise MfdArgumentError(L.MFD_ERR_ARG, f"x0 must hold as many values as b ({b.size}), got {e.size}")
         o = self._opts(tol, precision, maxit, inner_tol, check_every, use_graph, method)
         st = L.mfd_solve_stats()
         rc = lib().mfd_solve(self._h, ptr(b), ptr(e), nrhs, C.byref(o), C.byref(st))
@@ -596,6 +608,7 @@ class MaxwellOperator:
 
     def solve_adjoint(self, g, tol=1e-8, precision="fp64", maxit=None, method=None):
         g = L.carr(g)
+        self._one(g, "g")
         lam = np.zeros_like(g)
         o = self._opts(tol, precision, maxit, None, None, True, method)
         st = L.mfd_solve_stats()
@@ -605,16 +618,22 @@ class MaxwellOperator:
     def h_from_e(self, e, jm=None):
         """h_from_e (model.jl:276-279): h = (i/ω) Pμ⁻¹ (Cₑ e + jₘ), on the device."""
         e = L.carr(e)
+        self._one(e, "e")
         h = np.zeros_like(e)
         jm = None if jm is None else L.carr(jm)
+        if jm is not None:
+            self._one(jm, "jm")
         check(lib().mfd_h_from_e(self._h, ptr(e), None if jm is None else ptr(jm), ptr(h)))
         return h
 
     def e_from_h(self, h, je=None):
         """e_from_h (model.jl:281-284): e = (−i/ω) Pε⁻¹ (Cₘ h − jₑ), on the device (diagonal ε)."""
         h = L.carr(h)
+        self._one(h, "h")
         e = np.zeros_like(h)
         je = None if je is None else L.carr(je)
+        if je is not None:
+            self._one(je, "je")
         check(lib().mfd_e_from_h(self._h, ptr(h), None if je is None else ptr(je), ptr(e)))
         return e
 
@@ -634,6 +653,8 @@ class MaxwellOperator:
         """∂f/∂ε per E DOF: 2 Re(ω² λ ∘ e) for Aᵀλ = (∂f/∂e)ᵀ."""
         e = L.carr(e)
         lam = L.carr(lam)
+        self._one(e, "e")
+        self._one(lam, "lam")
         grad = np.zeros(e.size, dtype=np.float64)
         check(lib().mfd_adjoint_grad(self._h, ptr(e), ptr(lam), ptr(grad)))
         return grad

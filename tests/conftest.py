@@ -48,3 +48,4 @@ def pkg():
 def O():
     import oracle
     return oracle
+

@@ -574,12 +574,12 @@ def test_solve_streamed(precision, pkg, O):
     b = pkg.create_b(pkg.EE, om, m, je, jm)
     op = pkg.MaxwellOperator(m, om)
     say("streamed solve", precision)
-    e, st = op.solve(b, tol=1e-6, precision=precision)
+    e, st = op.solve(b, tol=1e-8, precision=precision)
     say("  iterations", st.iterations, "fp32", st.iterations_fp32, "relres", st.relres)
     op.close()
     A = O.to_scipy(O.create_A(spec_of(m), om, m.εarr, m.μarr))
     rel = np.linalg.norm(b - A @ e) / np.linalg.norm(b)
-    assert st.converged and rel <= 1.05e-6
+    assert st.converged and rel <= 1.05e-8
 
 
 def test_solve_bicgstab_symmetric(pkg, O):
@@ -620,6 +620,8 @@ def test_solve_c1_against_oracle(pkg, O):
     say("  mixed its", st8.iterations, "cycles", st8.outer_cycles, "res", rel8,
         "field err", np.linalg.norm(e8 - xo) / np.linalg.norm(xo))
     assert rel8 <= 1.05e-8
+    # fields to 1e-7 at residual 1e-8 (the drop-in contract), mixed precision included
+    assert np.linalg.norm(e8 - xo) / np.linalg.norm(xo) <= 1e-7
     op.close()
 
 

@@ -188,3 +188,52 @@ def test_cylinder_smoothing_bruteforce(O):
             checked += 1
     assert checked > 20
     assert ea[8, 8, 1, 2, 2] == e_in and ea[0, 0, 0, 2, 2] == 1.0
+
+
+def test_ball_offdiagonal_bruteforce(O):
+    """Each smoothed off-diagonal ε_vw of a ball, stored at the primal voxel
+    corners (the corner grid g₀ of boundft = EE) and averaged over the voxel
+    [i−½, i+½]×[j−½, j+½]×[k−½, k+½] whose corners are dual points, matches
+    Kottke's isotropic closed form (ε∥ − ε⊥ with ε∥ arithmetic and ε⊥ harmonic
+    in the fill fraction) times n_v n_w, with the fill fraction supersampled
+    and n the radial normal at the corner.  The tolerance is chosen so that a
+    flipped sign, or the normal components of another (v, w) pair, fail."""
+    N = (16, 16, 16)
+    s = spec(N)
+    R, e_in = 4.6, 12.0
+    c = np.array([8.3, 7.6, 8.15])
+    shapes = [("cuboid", (8, 8, 8), (16.0, 16.0, 16.0)), ("ball", tuple(c), R)]
+    I3 = np.eye(3)
+    ea, _ = O.matparams(s, shapes, [I3, e_in * I3], [I3, I3])
+    g = (np.arange(12) + 0.5) / 12
+    sub = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pairs = [(0, 1), (0, 2), (1, 2)]
+    tol = 0.025 * (e_in - 1)
+    worst = {"right": 0.0, "sign": 0.0, "pair": 0.0}
+    checked = 0
+    for i in range(1, N[0] - 1):
+        for j in range(1, N[1] - 1):
+            for k in range(1, N[2] - 1):
+                x0 = np.array([i, j, k], float)            # primal corner point
+                X = x0 - 0.5 + sub                         # its dual-cornered voxel
+                r = np.mean(np.sum((X - c) ** 2, axis=1) <= R * R)
+                if not 0.05 < r < 0.95:
+                    continue
+                n = (x0 - c) / np.linalg.norm(x0 - c)
+                epar = r * e_in + (1 - r)
+                eperp = 1 / (r / e_in + (1 - r))
+                for q, (v, w) in enumerate(pairs):
+                    got = ea[i, j, k, v, w].real
+                    ref = (eperp - epar) * n[v] * n[w]
+                    ov, ow = pairs[(q + 1) % 3]
+                    alt = (eperp - epar) * n[ov] * n[ow]   # the normal components of another pair
+                    worst["right"] = max(worst["right"], abs(got - ref))
+                    worst["sign"] = max(worst["sign"], abs(got + ref))
+                    worst["pair"] = max(worst["pair"], abs(got - alt))
+                    assert abs(got - ref) <= tol, (i, j, k, v, w, got, ref, r)
+                    assert abs(ea[i, j, k, w, v] - ea[i, j, k, v, w]) <= 1e-14
+                checked += 1
+    assert checked > 100
+    # the check discriminates: the plausible mistakes are far outside it
+    assert worst["sign"] > 4 * tol and worst["pair"] > 4 * tol, worst
+    assert worst["right"] < 0.5 * max(worst["sign"], worst["pair"])
