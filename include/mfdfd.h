@@ -210,6 +210,19 @@ int mfd_solve(mfd_op *op, const mfd_c64 *b, mfd_c64 *e, int64_t nrhs, const mfd_
 int mfd_solve_dev(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_solve_opts *o,
                   mfd_solve_stats *st);
 /* Adjoint solve Aᵀ λ = g using Aᵀ = S_E A S_E⁻¹ (complex symmetry). */
+/* mfd_solve_dev without allocating: no reference counterpart; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, method choice,
+ * convergence rules and results as mfd_solve_dev with use_graph = 0 (graphs
+ * are never used here), but the solver workspace lives in a fixed 64 MiB
+ * array per GPU that the library holds until the process ends (set up by
+ * mfd_ctx_create): about 150 bytes per unknown and right-hand side plus
+ * 1.5 MB per right-hand side, so a 32³ grid with 4 right-hand sides fits.
+ * A larger request fails with MFD_ERR_ARG before any GPU work.  Every call on
+ * a GPU shares that GPU's array: calls that overlap on one GPU race.  The
+ * operator must have been created (mfd_op_create allocates) beforehand. */
+int mfd_solve_dev_noalloc(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_solve_opts *o,
+                          mfd_solve_stats *st);
 /* Solve `count` independent systems A_p e_p = b_p, one RHS each (every
  * problem keeps its own grid, ε and ω; all operators share one context).
  * Each chunk of iterations of all unconverged problems is one hipGraph launch

@@ -27,6 +27,11 @@ int solve_adjoint_impl(mfd_op *op, const mfd_c64 *g, mfd_c64 *lam, const mfd_sol
 int adjoint_grad_impl(mfd_op *op, const mfd_c64 *e, const mfd_c64 *lam, double *grad);
 int create_b_impl(mfd_ctx *ctx, const mfd::DevGrid &G, int ft, mfd_c64 omega, const mfd_c64 *par, const mfd_c64 *je,
                   const mfd_c64 *jm, mfd_c64 *b);
+int solve_dev_noalloc_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_solve_opts *o,
+                           mfd_solve_stats *st);
+namespace mfd {
+int noalloc_scratch_init(int device);
+}
 
 using namespace mfd;
 
@@ -47,6 +52,9 @@ int mfd_ctx_create(int device, mfd_ctx **out) {
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) { set_error("no HIP device visible"); return MFD_ERR_HIP; }
     if (device < 0 || device >= n) { set_error("device index out of range"); return MFD_ERR_ARG; }
     MFD_HIP(hipSetDevice(device));
+    // device address of this GPU's scratch array (mfd_solve_dev_noalloc), taken
+    // once per device with that device current
+    MFD_TRY(noalloc_scratch_init(device));
     mfd_ctx *c = new (std::nothrow) mfd_ctx();
     if (!c) return MFD_ERR_ALLOC;
     c->device = device;
@@ -331,6 +339,12 @@ int mfd_solve_dev(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_so
     if (!op || !b || !e) { set_error("null argument"); return MFD_ERR_ARG; }
     CHECK_CTX(op->ctx);
     return solve_dev_impl(op, b, e, nrhs, o, st);
+}
+int mfd_solve_dev_noalloc(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_solve_opts *o,
+                          mfd_solve_stats *st) {
+    if (!op || !b || !e) { set_error("null argument"); return MFD_ERR_ARG; }
+    CHECK_CTX(op->ctx);
+    return solve_dev_noalloc_impl(op, b, e, nrhs, o, st);
 }
 int mfd_solve_batch(int count, mfd_op *const *ops, const mfd_c64 *const *b, mfd_c64 *const *e,
                     const mfd_solve_opts *o, mfd_solve_stats *st) {

@@ -101,6 +101,8 @@ SIGNATURES = {
     "mfd_solve_opts_default": (None, [C.POINTER(mfd_solve_opts)]),
     "mfd_solve": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(mfd_solve_opts), C.POINTER(mfd_solve_stats)]),
     "mfd_solve_dev": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(mfd_solve_opts), C.POINTER(mfd_solve_stats)]),
+    "mfd_solve_dev_noalloc": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(mfd_solve_opts),
+                                        C.POINTER(mfd_solve_stats)]),
     "mfd_solve_batch": (C.c_int, [C.c_int, _P, _P, _P, C.POINTER(mfd_solve_opts), _P]),
     "mfd_solve_batch_dev": (C.c_int, [C.c_int, _P, _P, _P, C.POINTER(mfd_solve_opts), _P]),
     "mfd_h_from_e": (C.c_int, [_P, _P, _P, _P]),

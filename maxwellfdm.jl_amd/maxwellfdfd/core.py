@@ -600,6 +600,18 @@ class MaxwellOperator:
             check(rc)
         return st
 
+    def solve_dev_noalloc(self, b_ptr, e_ptr, nrhs=1, tol=1e-8, precision="fp64", maxit=None, inner_tol=None,
+                          check_every=None, method=None, raise_on_noconv=True):
+        """solve_dev without any device allocation: the workspace lives in the
+        library's fixed per-GPU scratch array (see mfd_solve_dev_noalloc)."""
+        o = self._opts(tol, precision, maxit, inner_tol, check_every, False, method)
+        st = L.mfd_solve_stats()
+        rc = lib().mfd_solve_dev_noalloc(self._h, C.c_void_p(b_ptr), C.c_void_p(e_ptr), nrhs, C.byref(o),
+                                         C.byref(st))
+        if rc != L.MFD_OK and (raise_on_noconv or rc != L.MFD_ERR_NOCONV):
+            check(rc)
+        return st
+
     def solve_batch_dev(self, others, b_ptrs, e_ptrs, tol=1e-8, precision="fp64", maxit=None, inner_tol=None,
                         check_every=None, use_graph=True):
         """Device-pointer form of `solve_batch` with this operator first."""
