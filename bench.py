@@ -69,19 +69,24 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(m, om, fp64_iters, nthreads, iters):
-    """Oracle CPU FP64 path on a bounded sample: assembly of the CSC A plus
-    `iters` SQMR iterations with the CSC product; the per-iteration time is
-    scaled by the iteration count of the FP64 device solve."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
+def host_cores():
+    """Cores this process may use: the OpenMP share the host sets for it
+    (OMP_NUM_THREADS), else the CPUs it is allowed to run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env), "OMP_NUM_THREADS"
+    return len(os.sched_getaffinity(0)), "sched_getaffinity"
+
+
+def oracle_spec(m, om):
     import types
-    import oracle as O
     g = m.grid
-    spec = types.SimpleNamespace(lprim=g.lprim, isbloch=g.isbloch, boundft=(0, 0, 0), npml=m.Npml, omega_pml=om,
+    return types.SimpleNamespace(lprim=g.lprim, isbloch=g.isbloch, boundft=(0, 0, 0), npml=m.Npml, omega_pml=om,
                                  kbloch=(0, 0, 0), pml_m=4.0, pml_lnR=-16.0, order_cmpfirst=True)
-    shapes = []
-    et, mt = [], []
+
+
+def oracle_shapes(m):
+    shapes, et, mt = [], [], []
     for o in m.objs:
         s = o.shape
         if hasattr(s, "h"):
@@ -92,24 +97,66 @@ def cpu_baseline(m, om, fp64_iters, nthreads, iters):
             shapes.append(("cuboid", tuple(s.c), tuple(s.s)))
         et.append(o.mat.eps)
         mt.append(o.mat.mu)
+    return shapes, et, mt
+
+
+def cpu_baseline(m, om, fp64_iters, iters):
+    """CPU baselines on the GPU box's host cores, on bounded samples:
+    - the oracle's C FP64 path on the same system: CSC assembly (1 thread),
+      then `iters` SQMR iterations with an OpenMP CSR product, on every core
+      this process may use and on 1 thread; `value` is the all-core time per
+      iteration times the FP64 GPU solve's iteration count (an extrapolation,
+      labelled so);
+    - the reference's own `A \\ b`, stood in by scipy's SuperLU (one thread),
+      on the plain-method sphere generator at 20³ (24,000 unknowns): at the
+      32³ size a factorisation did not finish within 15 minutes."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import scipy.sparse.linalg as sla
+    import oracle as O
+    from maxwellfdfd import workloads as W
+    spec = oracle_spec(m, om)
+    shapes, et, mt = oracle_shapes(m)
     t0 = time.perf_counter()
     ea, ma = O.matparams(spec, shapes, np.array(et), np.array(mt))
     A = O.create_A(spec, om, ea, ma)
     t_asm = time.perf_counter() - t0
-    N = g.N
+    N = m.grid.N
     j = np.zeros(N + (3,), dtype=np.complex128, order="F")
     for src in getattr(m, "_bench_srcs", []):
         O.add_pointsrc(spec, 0, src[0], src[1], 1.0, j)
     je = O.field_arr2vec(spec, j)
     b = O.create_b(spec, om, ma, je, np.zeros_like(je))
+    ncores, how = host_cores()
+    per_it = {}
+    for nth in (ncores, 1):
+        t0 = time.perf_counter()
+        _, it, _ = O.solve_sqmr(spec, A, b, tol=1e-8, maxit=iters, nthreads=nth)
+        per_it[nth] = (time.perf_counter() - t0) / max(it, 1)
+    # the direct solve the reference's users run, at a size it finishes
+    md, omd = W.sphere_dipole(N=20, npml=5, r=3.75, lam=12.5)
+    sd = oracle_spec(md, omd)
+    shd, etd, mtd = oracle_shapes(md)
+    ead, mad = O.matparams(sd, shd, np.array(etd), np.array(mtd))
+    Ad = O.to_scipy(O.create_A(sd, omd, ead, mad)).tocsc()
+    jd = np.zeros((20, 20, 20, 3), dtype=np.complex128, order="F")
+    O.add_pointsrc(sd, 0, (10.0, 10.0, 10.0), (0.0, 0.0, 1.0), 1.0, jd)
+    bd = O.create_b(sd, omd, mad, O.field_arr2vec(sd, jd), np.zeros(Ad.shape[0], complex))
     t0 = time.perf_counter()
-    _, it, rr = O.solve_sqmr(spec, A, b, tol=1e-8, maxit=iters, nthreads=nthreads)
-    t_it = (time.perf_counter() - t0) / max(it, 1)
-    return dict(value=round(t_it * fp64_iters, 3), unit="s", cores=nthreads, kind="port",
-                sample=f"oracle C FP64 on the same {N[0]}x{N[1]}x{N[2]} system: CSC assembly "
-                       f"({t_asm:.1f} s, 1 thread) + {it} SQMR iterations (CSR SpMV, {nthreads} threads, "
-                       f"{t_it * 1e3:.1f} ms/iteration) scaled to the {fp64_iters} iterations of the FP64 GPU "
-                       f"solve", assembly_s=round(t_asm, 3), s_per_iteration=t_it)
+    xd = sla.splu(Ad).solve(bd)
+    t_lu = time.perf_counter() - t0
+    res_lu = float(np.linalg.norm(bd - Ad @ xd) / np.linalg.norm(bd))
+    return dict(value=round(per_it[ncores] * fp64_iters, 3), unit="s", cores=ncores, kind="port",
+                sample=f"oracle C FP64 on the same {N[0]}x{N[1]}x{N[2]} system: CSC assembly ({t_asm:.1f} s, "
+                       f"1 thread) + {iters} SQMR iterations (CSR SpMV) on {ncores} threads ({how}): "
+                       f"{per_it[ncores] * 1e3:.1f} ms/iteration; value = that time x the {fp64_iters} "
+                       f"iterations of the FP64 GPU solve (extrapolated, not run to 1e-8)",
+                assembly_s=round(t_asm, 3), s_per_iteration=per_it[ncores],
+                one_thread={"s_per_iteration": per_it[1], "solve_s_extrapolated": round(per_it[1] * fp64_iters, 3)},
+                direct_solve={"what": "scipy SuperLU factor + solve (1 thread), A \\ b stand-in, plain-method sphere "
+                                      "generator at 20^3 (24,000 unknowns; the 32^3 factorisation did not finish "
+                                      "in 15 min)",
+                              "s": round(t_lu, 3), "relres": res_lu})
 
 
 def kernel_table(op, cells, iters=50):
@@ -218,11 +265,53 @@ def run_uneven_batch(a):
             "iterations_sequential": its}
 
 
+def traced(fn):
+    """Runs fn() with the solver's trace on (MFD_SOLVE_TRACE: one line per
+    inner cycle and per true-residual probe, written by the library to fd 2)
+    and returns (fn's result, the trace lines)."""
+    import tempfile
+    os.environ["MFD_SOLVE_TRACE"] = "1"
+    sys.stderr.flush()
+    saved = os.dup(2)
+    with tempfile.TemporaryFile(mode="w+b") as f:
+        os.dup2(f.fileno(), 2)
+        try:
+            out = fn()
+        finally:
+            os.dup2(saved, 2)
+            os.close(saved)
+            del os.environ["MFD_SOLVE_TRACE"]
+        f.seek(0)
+        lines = f.read().decode("utf-8", "replace").splitlines()
+    for line in lines:
+        print(line, file=sys.stderr, flush=True)
+    return out, lines
+
+
+def residual_history(lines):
+    """(Krylov steps, FP64 relative residual) pairs from a solver trace: the
+    residual at each cycle start and at each true-residual probe."""
+    import re
+    hist, steps = [], 0
+    for line in lines:
+        m = re.search(r"\[mfd solve\] cycle (\d+) prec (\w+) residual ([0-9.e+-]+) target [0-9.e+-]+ iterations (\d+)", line)
+        if m:
+            hist.append((steps, float(m.group(3)), m.group(2)))
+            steps += int(m.group(4))
+            continue
+        m = re.search(r"\[mfd probe\] it (\d+) true residual ([0-9.e+-]+)", line)
+        if m:
+            hist.append((steps + int(m.group(1)), float(m.group(2)), "probe"))
+    return hist
+
+
 def run_large_aniso(a):
     """384×384×256 anisotropic case (200 smoothed spheres, full 3×3 ε, 8
     dipoles): assembly on the device, apply bandwidth of the full-tensor
-    stencil, and a bounded batched FP64 solve (a.maxit steps) with the
-    residual it reaches and the time per step."""
+    stencil, and a bounded batched solve (a.maxit steps, a.precision) with its
+    FP64 residual history, the time per step and the time to 1e-8 projected
+    from the history's last decade of progress."""
+    import math
     import numpy as np
     import torch
     import maxwellfdfd as M
@@ -237,27 +326,50 @@ def run_large_aniso(a):
     B = W.dipole_rhs(m, om, dips)
     op = M.MaxwellOperator(m, om)
     cells = m.grid.M
-    tiles = op.autotune(1, 5)
-    log("tiles", tiles)
+    t0 = time.perf_counter()
+    tiles = op.autotune(a.nrhs, 3)
+    log("tiles", tiles, "autotune", round(time.perf_counter() - t0, 1), "s")
     ms_apply = op.bench_apply(1, "fp64", iters=10)
     bpc = 6 * 16 + 6 * 16                   # x + y, and the six stored ε entries (3 diagonal + 3 corner)
     log("apply", round(ms_apply, 3), "ms")
+    it_ms = op.bench_iteration(a.nrhs, a.precision, 5)
+    log("iteration kernels", it_ms)
     dev = torch.device("cuda:0")
     b_d = torch.from_numpy(B).to(dev)
     e_d = torch.zeros_like(b_d)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    st = op.solve_dev(b_d.data_ptr(), e_d.data_ptr(), a.nrhs, tol=1e-8, maxit=a.maxit, raise_on_noconv=False)
-    torch.cuda.synchronize()
-    t_solve = time.perf_counter() - t0
+
+    def solve():
+        t0 = time.perf_counter()
+        st = op.solve_dev(b_d.data_ptr(), e_d.data_ptr(), a.nrhs, tol=1e-8, maxit=a.maxit, precision=a.precision,
+                          raise_on_noconv=False)
+        torch.cuda.synchronize()
+        return st, time.perf_counter() - t0
+
+    (st, t_solve), lines = traced(solve)
     op.close()
+    hist = residual_history(lines)
+    hist.append((int(st.iterations), float(st.relres), "end"))
+    ms_step = 1e3 * t_solve / max(st.iterations, 1)
+    proj = None
+    pts = [(k, r) for k, r, _ in hist if r > 0]
+    if not st.converged and len(pts) >= 2:
+        (k0, r0), (k1, r1) = pts[-2], pts[-1]
+        # steps per decade over the last stretch of the history
+        if r1 < r0 and k1 > k0:
+            spd = (k1 - k0) / math.log10(r0 / r1)
+            steps_needed = k1 + spd * math.log10(r1 / 1e-8)
+            proj = {"steps_per_decade": round(spd, 1), "steps_to_1e-8": int(steps_needed),
+                    "s_to_1e-8": round(steps_needed * ms_step * 1e-3, 1)}
     return {"workload": f"large-aniso {m.grid.N[0]}x{m.grid.N[1]}x{m.grid.N[2]}, 200 smoothed spheres "
-                        f"(full eps tensors: {full}), {a.nrhs} dipoles, FP64 SQMR bounded at {a.maxit} steps",
+                        f"(full eps tensors: {full}), {a.nrhs} dipoles, {a.precision} SQMR bounded at {a.maxit} steps",
             "cells": int(cells), "matparams_s": round(t_mat, 3), "tiles": [list(tiles[0]), list(tiles[1])],
             "apply_ms": round(ms_apply, 4),
             "apply_GBps": round(bpc * cells / (ms_apply * 1e-3) / 1e9, 1),
-            "solve_s": round(t_solve, 3), "steps": int(st.iterations), "relres_reached": st.relres,
-            "converged": bool(st.converged), "ms_per_step_all_rhs": round(1e3 * t_solve / max(st.iterations, 1), 3)}
+            "iteration_kernels_ms": [round(x, 4) for x in it_ms],
+            "solve_s": round(t_solve, 3), "steps": int(st.iterations), "steps_fp32": int(st.iterations_fp32),
+            "relres_reached": st.relres, "converged": bool(st.converged), "ms_per_step_all_rhs": round(ms_step, 3),
+            "residual_history": [[int(k), float(r), w] for k, r, w in hist], "projection": proj}
 
 
 def run_adjoint(a):
@@ -498,8 +610,7 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        nth = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(m, om, int(st_d.iterations), nth, a.cpu_iters)
+        cpu = cpu_baseline(m, om, int(st_d.iterations), a.cpu_iters)
 
     rdev = "float" if dom_prec == "fp32" else "double"
     out = {

@@ -43,6 +43,16 @@ enum { MFD_SQMR = 0, MFD_COCG = 1, MFD_BICGSTAB = 2 };
 
 typedef struct { double re, im; } mfd_c64;
 
+/* A Julia SparseMatrixCSC{ComplexF64,Int64} as it lies in memory: m × n with
+ * nnz stored entries, 1-based colptr (n + 1 values, colptr[0] = 1,
+ * colptr[n] = nnz + 1) and rowval (ascending within a column), nzval. */
+typedef struct {
+    int64_t m, n, nnz;
+    const int64_t *colptr;
+    const int64_t *rowval;
+    const mfd_c64 *nzval;
+} mfd_csc;
+
 /* Grid + Model boundary settings (MaxwellBase Grid(lprim, isbloch) and the
  * Model fields boundft/Npml/kbloch/ωpml/order_cmpfirst, model.jl:37-72). */
 typedef struct {
@@ -148,6 +158,30 @@ int mfd_A_nnz(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_
 int mfd_A_csc(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr,
               const mfd_c64 *mu_arr, int64_t *colptr, int64_t *rowval, mfd_c64 *nzval);
 
+/* create_A(ft, ω, Pε, Pμ, Cₑ, Cₘ) (model.jl:222-246) from the caller's
+ * assembled matrices, formed on the device: A = Cₘ (Pμ \ Cₑ) − ω² Pε (EE) or
+ * Cₑ (Pε \ Cₘ) − ω² Pμ (HH), with Julia SparseArrays' structure rules: the
+ * divisor must be diagonal (MFD_ERR_UNSUPPORTED otherwise, as model.jl:236
+ * notes), the product keeps its structural entries, and the subtraction keeps
+ * only nonzero results (skipped for ω = 0).  The same two phases as mfd_A_csc;
+ * the results equal mfd_A_csc's for the matrices mfd_paramop_csc and
+ * mfd_curl_csc return. */
+int mfd_A_from_csc_nnz(mfd_ctx *ctx, int ft, mfd_c64 omega, const mfd_csc *Pe, const mfd_csc *Pm, const mfd_csc *Ce,
+                       const mfd_csc *Cm, int64_t *nnz);
+int mfd_A_from_csc(mfd_ctx *ctx, int ft, mfd_c64 omega, const mfd_csc *Pe, const mfd_csc *Pm, const mfd_csc *Ce,
+                   const mfd_csc *Cm, int64_t *colptr, int64_t *rowval, mfd_c64 *nzval);
+/* create_b(ft, ω, Pε, Pμ, Cₑ, Cₘ, jₑ, jₘ) (model.jl:248-274) from the
+ * matrices: b = −Cₘ (Pμ \ jₘ) − iω jₑ (EE) or Cₑ (Pε \ jₑ) − iω jₘ (HH). */
+int mfd_b_from_csc(mfd_ctx *ctx, int ft, mfd_c64 omega, const mfd_csc *Pe, const mfd_csc *Pm, const mfd_csc *Ce,
+                   const mfd_csc *Cm, const mfd_c64 *je, const mfd_c64 *jm, mfd_c64 *b);
+/* h_from_e(e, ω, Pμ, Cₑ, jₘ) = (i/ω) (Pμ \ (Cₑ e + jₘ)) and
+ * e_from_h(h, ω, Pε, Cₘ, jₑ) = (−i/ω) (Pε \ (Cₘ h − jₑ)) (model.jl:276-284)
+ * from the matrices; jₘ / jₑ may be null (zero). */
+int mfd_h_from_e_csc(mfd_ctx *ctx, const mfd_c64 *e, mfd_c64 omega, const mfd_csc *Pm, const mfd_csc *Ce,
+                     const mfd_c64 *jm, mfd_c64 *h);
+int mfd_e_from_h_csc(mfd_ctx *ctx, const mfd_c64 *h, mfd_c64 omega, const mfd_csc *Pe, const mfd_csc *Cm,
+                     const mfd_c64 *je, mfd_c64 *e);
+
 /* create_b (model.jl:251-274): b = −Cₘ (Pμ \ jₘ) − iω jₑ for ft = EE (uses
  * mu_arr), b = Cₑ (Pε \ jₑ) − iω jₘ for ft = HH (uses eps_arr); any boundft. */
 int mfd_create_b(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr,
@@ -162,6 +196,15 @@ int mfd_create_b(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const m
  * ε (EE) needs boundft = EE on every axis, off-diagonal μ (HH) boundft = HH. */
 int mfd_op_create(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr,
                   const mfd_c64 *mu_arr, mfd_op **out);
+/* Operator from the caller's assembled A (the A of create_A / create_linsys,
+ * model.jl:209-246; n = 3 × cells): a row-major device copy applied by a
+ * sparse product, Jacobi-scaled, for mfd_op_apply(_dev) and the solvers.  If A
+ * equals its transpose it is complex-symmetric (SQMR by default, COCG, the
+ * adjoint solve); otherwise BiCGSTAB serves it (A from a PML grid is symmetric
+ * only up to the cell volumes, which A alone does not carry).  The matrix-free
+ * mfd_op_create is the fast form of the same operator.  Tile and field
+ * conversion calls do not apply to it (MFD_ERR_UNSUPPORTED). */
+int mfd_op_create_csc(mfd_ctx *ctx, const mfd_csc *A, mfd_op **out);
 int mfd_op_destroy(mfd_op *op);
 /* y = A x for nrhs vectors stored one after another (host pointers). */
 int mfd_op_apply(mfd_op *op, const mfd_c64 *x, mfd_c64 *y, int64_t nrhs);
@@ -177,8 +220,10 @@ int mfd_op_apply_dev(mfd_op *op, const void *x, void *y, int64_t nrhs, int prec)
 int mfd_op_set_graddiv(mfd_op *op, int enable);
 /* Tile-shape autotuner for the stencil (rows per block TY, z-planes per block
  * KZ, tile order: 1 = each XCD gets a contiguous run of tiles, 0 = dispatch
- * order): times each candidate for nrhs right-hand sides and keeps the fastest
- * per precision.  Outputs {TY, KZ, order} may be null.  No reference counterpart. */
+ * order): times each candidate inside complete solver iterations for nrhs
+ * right-hand sides (iters of them, on the solver workspace) and keeps the one
+ * with the fastest iteration per precision; ms[] is that iteration time.
+ * Outputs may be null.  No reference counterpart. */
 int mfd_op_autotune(mfd_op *op, int64_t nrhs, int iters, int tile64[3], int tile32[3], double ms[2]);
 /* Fix the tile shape by hand: TY in {1,2,4,8} (0 = cache-reliant direct
  * kernel), KZ >= 1, xcd_order 0 or 1.  Until this or mfd_op_autotune is

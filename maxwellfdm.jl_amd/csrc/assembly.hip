@@ -442,4 +442,389 @@ int A_csc(mfd_ctx *ctx, const DevGrid &G, int ft, mfd_c64 omega, const mfd_c64 *
                      [&](int64_t *cp, int64_t *rv, c64 *nz) { k_A<true><<<nblk(n, 128), 128, 0, s>>>(g, ft, pa, da, full, om2, omz, n, nullptr, cp, rv, nz); });
 }
 
+
+// ------------------------------------------ products of caller-given CSC matrices
+//
+// create_A / create_b / h_from_e / e_from_h of the reference (model.jl:222-284)
+// take the assembled matrices (Pε, Pμ, Cₑ, Cₘ).  These entry points take them
+// as Julia hands them over (SparseMatrixCSC{ComplexF64,Int64}: 1-based colptr
+// and rowval, rows ascending within a column) and form the same results on the
+// device with Julia SparseArrays' structure rules: the divided matrix must be
+// diagonal (`Pμ \ Cₑ` scales the rows of Cₑ), the product keeps every
+// structural entry (Gustavson's column-by-column product, contributions to an
+// entry added in ascending inner index), and `A - ω² Pε` keeps only nonzero
+// results, which Julia's sparse map does; with ω = 0 the subtraction is
+// skipped.
+
+DevCsc::~DevCsc() { if (buf) hipFree(buf); }
+
+// per column: colptr nondecreasing, rows in [1, m] and strictly ascending
+__global__ void k_csc_check(int64_t m, int64_t n, const int64_t *cp, const int64_t *rv, int *flag) {
+    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int64_t a = cp[j] - 1, b = cp[j + 1] - 1;
+    if (b < a) { atomicOr(flag, 1); return; }
+    int64_t prev = 0;
+    for (int64_t k = a; k < b; ++k) {
+        const int64_t r = rv[k];
+        if (r < 1 || r > m || r <= prev) { atomicOr(flag, 1); return; }
+        prev = r;
+    }
+}
+
+int csc_upload(mfd_ctx *ctx, const mfd_csc *h, const char *name, DevCsc &d) {
+    if (!h || !h->colptr || (h->nnz > 0 && (!h->rowval || !h->nzval)) || h->m < 0 || h->n < 0 || h->nnz < 0) {
+        set_error(std::string(name) + ": null or negative CSC fields");
+        return MFD_ERR_ARG;
+    }
+    if (h->colptr[0] != 1 || h->colptr[h->n] != h->nnz + 1) {
+        set_error(std::string(name) + ": colptr must start at 1 and end at nnz + 1 (1-based, Julia SparseMatrixCSC)");
+        return MFD_ERR_ARG;
+    }
+    hipStream_t s = ctx->stream;
+    const size_t bcp = sizeof(int64_t) * (h->n + 1), brv = sizeof(int64_t) * h->nnz, bnz = sizeof(c64) * h->nnz;
+    const size_t off_rv = (bcp + 255) / 256 * 256, off_nz = off_rv + (brv + 255) / 256 * 256;
+    const size_t off_fl = off_nz + (bnz + 255) / 256 * 256;
+    MFD_HIP(hipMalloc(&d.buf, off_fl + 256));
+    d.m = h->m;
+    d.n = h->n;
+    d.nnz = h->nnz;
+    d.colptr = (int64_t *)d.buf;
+    d.rowval = (int64_t *)((char *)d.buf + off_rv);
+    d.nzval = (c64 *)((char *)d.buf + off_nz);
+    int *flag = (int *)((char *)d.buf + off_fl);
+    MFD_HIP(hipMemcpyAsync(d.colptr, h->colptr, bcp, hipMemcpyHostToDevice, s));
+    if (h->nnz) {
+        MFD_HIP(hipMemcpyAsync(d.rowval, h->rowval, brv, hipMemcpyHostToDevice, s));
+        MFD_HIP(hipMemcpyAsync(d.nzval, h->nzval, bnz, hipMemcpyHostToDevice, s));
+    }
+    MFD_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
+    if (h->n) k_csc_check<<<nblk(h->n, 256), 256, 0, s>>>(h->m, h->n, d.colptr, d.rowval, flag);
+    MFD_HIP(hipGetLastError());
+    int bad = 0;
+    MFD_HIP(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    MFD_HIP(hipStreamSynchronize(s));
+    if (bad) {
+        set_error(std::string(name) + ": colptr must not decrease and rows must be in 1..m, ascending within a column");
+        return MFD_ERR_ARG;
+    }
+    return MFD_OK;
+}
+
+// diagonal of a CSC matrix that must be diagonal (the divisor of `P \ C`):
+// flag bit 0 = an entry off the diagonal, bit 1 = a zero or missing diagonal entry
+__global__ void k_csc_diag(DevCscV P, c64 *d, int *flag) {
+    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= P.n) return;
+    c64 v = mk<double>(0, 0);
+    for (int64_t k = P.colptr[j] - 1; k < P.colptr[j + 1] - 1; ++k) {
+        if (P.rowval[k] - 1 != j) atomicOr(flag, 1);
+        else v = P.nzval[k];
+    }
+    if (iszero(v)) atomicOr(flag, 2);
+    d[j] = v;
+}
+
+constexpr int SPGEMM_MAXC = 64;   // distinct rows per product column handled by one thread
+
+// Column j of R = X·(D⁻¹Y), then R − s2·S when sub (numerical zeros dropped);
+// count pass (FILL false) or fill pass.  flag bit 2: a column needed more than
+// SPGEMM_MAXC distinct rows.
+template <bool FILL>
+__global__ __launch_bounds__(128) void k_spgemm(DevCscV X, DevCscV Y, const c64 *__restrict__ dinv_div,
+                                                DevCscV S, c64 s2, int sub, int64_t n, int64_t *cnt,
+                                                const int64_t *colptr, int64_t *rowval, c64 *nzval, int *flag) {
+    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int64_t rows[SPGEMM_MAXC];
+    c64 acc[SPGEMM_MAXC];
+    int nc = 0;
+    for (int64_t kb = Y.colptr[j] - 1; kb < Y.colptr[j + 1] - 1; ++kb) {
+        const int64_t k = Y.rowval[kb] - 1;
+        const c64 bv = cdiv(Y.nzval[kb], dinv_div[k]);   // (P \ C)[k, j]: row k of C divided by P[k, k]
+        for (int64_t ka = X.colptr[k] - 1; ka < X.colptr[k + 1] - 1; ++ka) {
+            const int64_t i = X.rowval[ka] - 1;
+            const c64 v = X.nzval[ka] * bv;
+            int x = 0;
+            while (x < nc && rows[x] != i) ++x;
+            if (x < nc) acc[x] = acc[x] + v;
+            else if (nc < SPGEMM_MAXC) { rows[nc] = i; acc[nc] = v; ++nc; }
+            else {   // refused by the caller: the column contributes no entries
+                atomicOr(flag, 4);
+                if (!FILL) cnt[j] = 0;
+                return;
+            }
+        }
+    }
+    for (int x = 1; x < nc; ++x) {   // rows ascending
+        const int64_t r = rows[x];
+        const c64 a = acc[x];
+        int y = x - 1;
+        while (y >= 0 && rows[y] > r) { rows[y + 1] = rows[y]; acc[y + 1] = acc[y]; --y; }
+        rows[y + 1] = r;
+        acc[y + 1] = a;
+    }
+    int64_t o = FILL ? colptr[j] : 0, m = 0;
+    if (!sub) {
+        for (int x = 0; x < nc; ++x) {
+            if (FILL) { rowval[o + m] = rows[x] + 1; nzval[o + m] = acc[x]; }
+            ++m;
+        }
+    } else {
+        int x = 0;
+        int64_t b = S.colptr[j] - 1;
+        const int64_t be = S.colptr[j + 1] - 1;
+        while (x < nc || b < be) {
+            const int64_t ra = x < nc ? rows[x] : INT64_MAX, rb = b < be ? S.rowval[b] - 1 : INT64_MAX;
+            c64 v;
+            int64_t r;
+            if (ra == rb) { v = acc[x] - s2 * S.nzval[b]; r = ra; ++x; ++b; }
+            else if (ra < rb) { v = acc[x]; r = ra; ++x; }
+            else { v = mk<double>(0, 0) - s2 * S.nzval[b]; r = rb; ++b; }
+            if (!iszero(v)) {
+                if (FILL) { rowval[o + m] = r + 1; nzval[o + m] = v; }
+                ++m;
+            }
+        }
+    }
+    if (!FILL) cnt[j] = m;
+}
+
+// ---- transpose to row-major (CSR) with rows' columns ascending
+
+__global__ void k_rowcount(DevCscV A, int64_t *rc) {
+    int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e < A.nnz) atomicAdd((unsigned long long *)&rc[A.rowval[e] - 1], 1ull);
+}
+__global__ void k_csr_scatter(DevCscV A, const int64_t *rowptr, int64_t *fillc, int32_t *colind, c64 *val) {
+    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= A.n) return;
+    for (int64_t k = A.colptr[j] - 1; k < A.colptr[j + 1] - 1; ++k) {
+        const int64_t i = A.rowval[k] - 1;
+        const int64_t pos = rowptr[i] + (int64_t)atomicAdd((unsigned long long *)&fillc[i], 1ull);
+        colind[pos] = (int32_t)j;
+        val[pos] = A.nzval[k];
+    }
+}
+__global__ void k_csr_sortrows(int64_t m, const int64_t *rowptr, int32_t *colind, c64 *val) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int64_t a = rowptr[i], b = rowptr[i + 1];
+    for (int64_t x = a + 1; x < b; ++x) {
+        const int32_t c = colind[x];
+        const c64 v = val[x];
+        int64_t y = x - 1;
+        while (y >= a && colind[y] > c) { colind[y + 1] = colind[y]; val[y + 1] = val[y]; --y; }
+        colind[y + 1] = c;
+        val[y + 1] = v;
+    }
+}
+
+DevCsr::~DevCsr() { if (buf) hipFree(buf); }
+
+int csr_of(mfd_ctx *ctx, const DevCsc &A, DevCsr &R) {
+    if (A.n > INT32_MAX) { set_error("matrix too wide for 32-bit column indices"); return MFD_ERR_ARG; }
+    hipStream_t s = ctx->stream;
+    const int64_t m = A.m, nnz = A.nnz;
+    const size_t brp = sizeof(int64_t) * (m + 1), bci = sizeof(int32_t) * nnz, bv = sizeof(c64) * nnz;
+    const size_t off_ci = (brp + 255) / 256 * 256, off_v = off_ci + (bci + 255) / 256 * 256;
+    const size_t off_c = off_v + (bv + 255) / 256 * 256;
+    MFD_HIP(hipMalloc(&R.buf, off_c + brp + 256));
+    R.m = m;
+    R.n = A.n;
+    R.nnz = nnz;
+    R.rowptr = (int64_t *)R.buf;
+    R.colind = (int32_t *)((char *)R.buf + off_ci);
+    R.val = (c64 *)((char *)R.buf + off_v);
+    int64_t *rc = (int64_t *)((char *)R.buf + off_c);   // row counts, then fill counters
+    MFD_HIP(hipMemsetAsync(rc, 0, brp, s));
+    if (nnz) k_rowcount<<<nblk(nnz, 256), 256, 0, s>>>(A.view(), rc);
+    size_t tmpb = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, tmpb, rc, R.rowptr, m + 1, s);
+    void *tmp = nullptr;
+    MFD_HIP(hipMalloc(&tmp, tmpb ? tmpb : 1));
+    hipcub::DeviceScan::ExclusiveSum(tmp, tmpb, rc, R.rowptr, m + 1, s);
+    hipError_t e = hipStreamSynchronize(s);
+    hipFree(tmp);
+    MFD_HIP(e);
+    MFD_HIP(hipMemsetAsync(rc, 0, brp, s));
+    if (A.n) k_csr_scatter<<<nblk(A.n, 256), 256, 0, s>>>(A.view(), R.rowptr, rc, R.colind, R.val);
+    if (m) k_csr_sortrows<<<nblk(m, 256), 256, 0, s>>>(m, R.rowptr, R.colind, R.val);
+    MFD_HIP(hipGetLastError());
+    MFD_HIP(hipStreamSynchronize(s));
+    return MFD_OK;
+}
+
+// y = A v, one thread per row, columns ascending (the order of a column-by-
+// column CSC product into y); then y = fy·(y + add) / div elementwise, each
+// step only where its operand is given (fy applied last)
+__global__ void k_csr_mv(int64_t m, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
+                         const c64 *__restrict__ val, const c64 *__restrict__ v, const c64 *__restrict__ add,
+                         c64 addsign, const c64 *__restrict__ div, c64 fy, int scale, c64 *__restrict__ y) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    c64 s = mk<double>(0, 0);
+    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) s = s + val[k] * v[colind[k]];
+    if (add) s = s + addsign * add[i];
+    if (div) s = cdiv(s, div[i]);
+    if (scale) s = fy * s;
+    y[i] = s;
+}
+
+__global__ void k_divdiag(int64_t n, const c64 *a, const c64 *d, c64 *out) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = cdiv(a[i], d[i]);
+}
+
+// b .*= -1 (when neg); b .-= iω .* j unless ω = 0
+__global__ void k_b_finish(int64_t n, c64 *b, int neg, c64 iw, int omz, const c64 *j) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    c64 v = b[i];
+    if (neg) v = v * mk<double>(-1.0, 0.0);
+    if (!omz) v = v - iw * j[i];
+    b[i] = v;
+}
+
+static int square_n(const mfd_csc *const *ms, const char *const *names, int cnt, int64_t *n) {
+    *n = -1;
+    for (int k = 0; k < cnt; ++k) {
+        if (!ms[k]) { set_error(std::string(names[k]) + ": null matrix"); return MFD_ERR_ARG; }
+        if (ms[k]->m != ms[k]->n || (*n >= 0 && ms[k]->n != *n)) {
+            set_error("the matrices must be square and of one size n (3 unknowns per cell)");
+            return MFD_ERR_ARG;
+        }
+        *n = ms[k]->n;
+    }
+    return MFD_OK;
+}
+
+// diagonal of the divisor `P` of `P \ C`, refused unless diagonal and nonsingular
+static int divisor_diag(mfd_ctx *ctx, const DevCsc &P, const char *what, DevArray &d) {
+    hipStream_t s = ctx->stream;
+    MFD_HIP(hipMalloc(&d.p, sizeof(c64) * P.n + 256));
+    int *flag = (int *)((char *)d.p + sizeof(c64) * P.n);
+    MFD_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
+    if (P.n) k_csc_diag<<<nblk(P.n, 256), 256, 0, s>>>(P.view(), (c64 *)d.p, flag);
+    int f = 0;
+    MFD_HIP(hipMemcpyAsync(&f, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    MFD_HIP(hipStreamSynchronize(s));
+    if (f & 1) {
+        set_error(std::string(what) + " must be diagonal (division unsupported otherwise, model.jl:236, 239)");
+        return MFD_ERR_UNSUPPORTED;
+    }
+    if (f & 2) { set_error(std::string(what) + " has a zero or missing diagonal entry (singular)"); return MFD_ERR_ARG; }
+    return MFD_OK;
+}
+
+int A_from_csc(mfd_ctx *ctx, int ft, mfd_c64 omega, const mfd_csc *Pe, const mfd_csc *Pm, const mfd_csc *Ce,
+               const mfd_csc *Cm, int64_t *nnz, int64_t *colptr, int64_t *rowval, mfd_c64 *nzval) {
+    if (ft != EE && ft != HH) { set_error("create_A: ft must be EE or HH (model.jl:241-245)"); return MFD_ERR_UNSUPPORTED; }
+    const mfd_csc *ms[4] = {Pe, Pm, Ce, Cm};
+    const char *names[4] = {"Pε", "Pμ", "Cₑ", "Cₘ"};
+    int64_t n = 0;
+    MFD_TRY(square_n(ms, names, 4, &n));
+    // EE: A = Cₘ (Pμ \ Cₑ) − ω² Pε;  HH: A = Cₑ (Pε \ Cₘ) − ω² Pμ
+    const mfd_csc *X = ft == EE ? Cm : Ce, *Y = ft == EE ? Ce : Cm, *D = ft == EE ? Pm : Pe, *S = ft == EE ? Pe : Pm;
+    DevCsc dX, dY, dD, dS;
+    MFD_TRY(csc_upload(ctx, X, ft == EE ? "Cₘ" : "Cₑ", dX));
+    MFD_TRY(csc_upload(ctx, Y, ft == EE ? "Cₑ" : "Cₘ", dY));
+    MFD_TRY(csc_upload(ctx, D, ft == EE ? "Pμ" : "Pε", dD));
+    MFD_TRY(csc_upload(ctx, S, ft == EE ? "Pε" : "Pμ", dS));
+    DevArray dd;
+    MFD_TRY(divisor_diag(ctx, dD, ft == EE ? "Pμ" : "Pε", dd));
+    hipStream_t s = ctx->stream;
+    const c64 om = mk<double>(omega.re, omega.im), s2 = om * om;
+    const int sub = !(omega.re == 0.0 && omega.im == 0.0);
+    int *flag = nullptr;
+    MFD_HIP(hipMalloc(&flag, sizeof(int)));
+    DevArray fguard;
+    fguard.p = flag;
+    MFD_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
+    const c64 *dv = (const c64 *)dd.p;
+    int rc = csc_drive(s, n, nnz, colptr, rowval, nzval,
+                       [&](int64_t *cnt) { k_spgemm<false><<<nblk(n, 128), 128, 0, s>>>(dX.view(), dY.view(), dv, dS.view(), s2, sub, n, cnt, nullptr, nullptr, nullptr, flag); },
+                       [&](int64_t *cp, int64_t *rv, c64 *nz) { k_spgemm<true><<<nblk(n, 128), 128, 0, s>>>(dX.view(), dY.view(), dv, dS.view(), s2, sub, n, nullptr, cp, rv, nz, flag); });
+    int f = 0;
+    MFD_HIP(hipMemcpyAsync(&f, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    MFD_HIP(hipStreamSynchronize(s));
+    if (f & 4) {
+        set_error("create_A: a column of the product has more than 64 distinct rows (not a curl-curl operator)");
+        return MFD_ERR_UNSUPPORTED;
+    }
+    return rc;
+}
+
+int b_from_csc(mfd_ctx *ctx, int ft, mfd_c64 omega, const mfd_csc *Pe, const mfd_csc *Pm, const mfd_csc *Ce,
+               const mfd_csc *Cm, const mfd_c64 *je, const mfd_c64 *jm, mfd_c64 *b) {
+    if (ft != EE && ft != HH) { set_error("create_b: ft must be EE or HH (model.jl:269-271)"); return MFD_ERR_UNSUPPORTED; }
+    if (!je || !jm || !b) { set_error("create_b: null jₑ, jₘ or b"); return MFD_ERR_ARG; }
+    const mfd_csc *ms[4] = {Pe, Pm, Ce, Cm};
+    const char *names[4] = {"Pε", "Pμ", "Cₑ", "Cₘ"};
+    int64_t n = 0;
+    MFD_TRY(square_n(ms, names, 4, &n));
+    // EE: b = −Cₘ (Pμ \ jₘ) − iω jₑ;  HH: b = Cₑ (Pε \ jₑ) − iω jₘ
+    const mfd_csc *X = ft == EE ? Cm : Ce, *D = ft == EE ? Pm : Pe;
+    const mfd_c64 *jd = ft == EE ? jm : je, *jw = ft == EE ? je : jm;
+    DevCsc dX, dD;
+    MFD_TRY(csc_upload(ctx, X, ft == EE ? "Cₘ" : "Cₑ", dX));
+    MFD_TRY(csc_upload(ctx, D, ft == EE ? "Pμ" : "Pε", dD));
+    DevArray dd;
+    MFD_TRY(divisor_diag(ctx, dD, ft == EE ? "Pμ" : "Pε", dd));
+    DevCsr R;
+    MFD_TRY(csr_of(ctx, dX, R));
+    hipStream_t s = ctx->stream;
+    DevArray V;
+    MFD_HIP(hipMalloc(&V.p, sizeof(c64) * 3 * n + 64));
+    c64 *y = (c64 *)V.p, *jdd = y + n, *jwd = y + 2 * n;
+    MFD_HIP(hipMemcpyAsync(jdd, jd, sizeof(c64) * n, hipMemcpyHostToDevice, s));
+    MFD_HIP(hipMemcpyAsync(jwd, jw, sizeof(c64) * n, hipMemcpyHostToDevice, s));
+    const c64 om = mk<double>(omega.re, omega.im), iw = om * mk<double>(0.0, 1.0);
+    const int omz = omega.re == 0.0 && omega.im == 0.0;
+    if (n) {
+        k_divdiag<<<nblk(n, 256), 256, 0, s>>>(n, jdd, (const c64 *)dd.p, jdd);
+        k_csr_mv<<<nblk(n, 256), 256, 0, s>>>(n, R.rowptr, R.colind, R.val, jdd, nullptr, mk<double>(1, 0), nullptr,
+                                               mk<double>(1, 0), 0, y);
+        k_b_finish<<<nblk(n, 256), 256, 0, s>>>(n, y, ft == EE, iw, omz, jwd);
+    }
+    MFD_HIP(hipGetLastError());
+    MFD_HIP(hipMemcpyAsync(b, y, sizeof(c64) * n, hipMemcpyDeviceToHost, s));
+    MFD_HIP(hipStreamSynchronize(s));
+    return MFD_OK;
+}
+
+int field_from_csc(mfd_ctx *ctx, int which, mfd_c64 omega, const mfd_csc *P, const mfd_csc *Cx,
+                   const mfd_c64 *in, const mfd_c64 *src, mfd_c64 *out) {
+    const char *pn = which == 0 ? "Pμ" : "Pε", *cn = which == 0 ? "Cₑ" : "Cₘ";
+    if (!in || !out) { set_error("null field vector"); return MFD_ERR_ARG; }
+    if (omega.re == 0.0 && omega.im == 0.0) { set_error("h_from_e/e_from_h: ω must be nonzero"); return MFD_ERR_ARG; }
+    const mfd_csc *ms[2] = {P, Cx};
+    const char *names[2] = {pn, cn};
+    int64_t n = 0;
+    MFD_TRY(square_n(ms, names, 2, &n));
+    DevCsc dP, dC;
+    MFD_TRY(csc_upload(ctx, P, pn, dP));
+    MFD_TRY(csc_upload(ctx, Cx, cn, dC));
+    DevArray dd;
+    MFD_TRY(divisor_diag(ctx, dP, pn, dd));
+    DevCsr R;
+    MFD_TRY(csr_of(ctx, dC, R));
+    hipStream_t s = ctx->stream;
+    DevArray V;
+    MFD_HIP(hipMalloc(&V.p, sizeof(c64) * 3 * n + 64));
+    c64 *vin = (c64 *)V.p, *vsrc = vin + n, *vout = vin + 2 * n;
+    MFD_HIP(hipMemcpyAsync(vin, in, sizeof(c64) * n, hipMemcpyHostToDevice, s));
+    if (src) MFD_HIP(hipMemcpyAsync(vsrc, src, sizeof(c64) * n, hipMemcpyHostToDevice, s));
+    // h = (i/ω) (Pμ \ (Cₑ e + jₘ));  e = (−i/ω) (Pε \ (Cₘ h − jₑ))
+    const c64 om = mk<double>(omega.re, omega.im);
+    const c64 f = cdiv(mk<double>(0.0, which == 0 ? 1.0 : -1.0), om);
+    if (n)
+        k_csr_mv<<<nblk(n, 256), 256, 0, s>>>(n, R.rowptr, R.colind, R.val, vin, src ? vsrc : nullptr,
+                                               mk<double>(which == 0 ? 1.0 : -1.0, 0), (const c64 *)dd.p, f, 1, vout);
+    MFD_HIP(hipGetLastError());
+    MFD_HIP(hipMemcpyAsync(out, vout, sizeof(c64) * n, hipMemcpyDeviceToHost, s));
+    MFD_HIP(hipStreamSynchronize(s));
+    return MFD_OK;
+}
+
 }  // namespace mfd

@@ -27,6 +27,7 @@ int solve_adjoint_impl(mfd_op *op, const mfd_c64 *g, mfd_c64 *lam, const mfd_sol
 int adjoint_grad_impl(mfd_op *op, const mfd_c64 *e, const mfd_c64 *lam, double *grad);
 int create_b_impl(mfd_ctx *ctx, const mfd::DevGrid &G, int ft, mfd_c64 omega, const mfd_c64 *par, const mfd_c64 *je,
                   const mfd_c64 *jm, mfd_c64 *b);
+int op_create_csc_impl(mfd_ctx *ctx, const mfd_csc *A, mfd_op **out);
 int solve_dev_noalloc_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_solve_opts *o,
                            mfd_solve_stats *st);
 namespace mfd {
@@ -294,6 +295,39 @@ int mfd_op_create(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const 
     return op_create_impl(ctx, g, ft, omega, eps_arr, mu_arr, out);
 }
 int mfd_op_destroy(mfd_op *op) { return op_destroy_impl(op); }
+int mfd_op_create_csc(mfd_ctx *ctx, const mfd_csc *A, mfd_op **out) {
+    CHECK_CTX(ctx);
+    if (!out || !A) { set_error("null argument"); return MFD_ERR_ARG; }
+    return op_create_csc_impl(ctx, A, out);
+}
+int mfd_A_from_csc_nnz(mfd_ctx *ctx, int ft, mfd_c64 omega, const mfd_csc *Pe, const mfd_csc *Pm, const mfd_csc *Ce,
+                       const mfd_csc *Cm, int64_t *nnz) {
+    CHECK_CTX(ctx);
+    if (!nnz) { set_error("null argument"); return MFD_ERR_ARG; }
+    return A_from_csc(ctx, ft, omega, Pe, Pm, Ce, Cm, nnz, nullptr, nullptr, nullptr);
+}
+int mfd_A_from_csc(mfd_ctx *ctx, int ft, mfd_c64 omega, const mfd_csc *Pe, const mfd_csc *Pm, const mfd_csc *Ce,
+                   const mfd_csc *Cm, int64_t *colptr, int64_t *rowval, mfd_c64 *nzval) {
+    CHECK_CTX(ctx);
+    if (!colptr || !rowval || !nzval) { set_error("null output"); return MFD_ERR_ARG; }
+    int64_t nnz = 0;
+    return A_from_csc(ctx, ft, omega, Pe, Pm, Ce, Cm, &nnz, colptr, rowval, nzval);
+}
+int mfd_b_from_csc(mfd_ctx *ctx, int ft, mfd_c64 omega, const mfd_csc *Pe, const mfd_csc *Pm, const mfd_csc *Ce,
+                   const mfd_csc *Cm, const mfd_c64 *je, const mfd_c64 *jm, mfd_c64 *b) {
+    CHECK_CTX(ctx);
+    return b_from_csc(ctx, ft, omega, Pe, Pm, Ce, Cm, je, jm, b);
+}
+int mfd_h_from_e_csc(mfd_ctx *ctx, const mfd_c64 *e, mfd_c64 omega, const mfd_csc *Pm, const mfd_csc *Ce,
+                     const mfd_c64 *jm, mfd_c64 *h) {
+    CHECK_CTX(ctx);
+    return field_from_csc(ctx, 0, omega, Pm, Ce, e, jm, h);
+}
+int mfd_e_from_h_csc(mfd_ctx *ctx, const mfd_c64 *h, mfd_c64 omega, const mfd_csc *Pe, const mfd_csc *Cm,
+                     const mfd_c64 *je, mfd_c64 *e) {
+    CHECK_CTX(ctx);
+    return field_from_csc(ctx, 1, omega, Pe, Cm, h, je, e);
+}
 int mfd_op_apply(mfd_op *op, const mfd_c64 *x, mfd_c64 *y, int64_t nrhs) {
     if (!op) { set_error("null operator"); return MFD_ERR_ARG; }
     CHECK_CTX(op->ctx);

@@ -78,6 +78,10 @@ struct DGrid {
     const cx<R> *sdlinv[2][3];  // 1 / stretched ∆l
     const cx<R> *ssq[2][3];     // principal sqrt of stretched ∆l
     const cx<R> *ssqinv[2][3];  // 1 / sqrt
+    // per-DOF T and T⁻¹ ([3p + c]) where the metric is not separable (an
+    // operator built from an assembled matrix); null: T from the tables above
+    const cx<R> *tdof = nullptr;
+    const cx<R> *tdofinv = nullptr;
 };
 
 __host__ __device__ inline int g0type(const int *bft, int ft, int a) {

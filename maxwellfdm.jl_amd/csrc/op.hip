@@ -51,6 +51,10 @@ __device__ inline void dof2cp(const DGrid<R> &g, int64_t e, int &c, int64_t &p) 
 // type (which each axis's boundft sets)
 template <typename R>
 __device__ inline cx<R> tfac(const DGrid<R> &g, int c, int i, int j, int k, bool inv) {
+    if (g.tdof) {
+        const int64_t p = i + (int64_t)g.N[0] * (j + (int64_t)g.N[1] * k);
+        return inv ? g.tdofinv[3 * p + c] : g.tdof[3 * p + c];
+    }
     const cx<R> *const(*t)[3] = inv ? g.ssqinv : g.ssq;
     cx<R> a = t[gtype(g.bft, EE, c, 0)][0][i];
     cx<R> b = t[gtype(g.bft, EE, c, 1)][1][j];
@@ -422,6 +426,144 @@ __global__ __launch_bounds__(BS) void k_apply(DGrid<R> g, OpData<R> od, const cx
     block_partial<2>(acc, part + (size_t)rhs * NPART * 4, (int)blockIdx.x);
 }
 
+// Sparse product for an operator built from a caller's CSC matrix A (stored
+// row-major): MODE_A y = A x, MODE_P y = Q T A T⁻¹ Q x with qin = Q T⁻¹ and
+// qout = Q T per DOF (Q = diag(A)^-1/2, T the symmetrizing scaling); the
+// epilogues are those of k_apply.
+// LPR lanes share a row (its entries are contiguous), summed by shuffles.
+template <typename R, int MODE, int EPI>
+__global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int64_t *__restrict__ rowptr,
+                                             const int32_t *__restrict__ colind, const c64 *__restrict__ val,
+                                             const c64 *__restrict__ qin, const c64 *__restrict__ qout,
+                                             const cx<R> *__restrict__ xall,
+                                             cx<R> *__restrict__ yall, const c64 *__restrict__ ball, int64_t vs,
+                                             double *part, const Scal *scal, const int *__restrict__ mask) {
+    constexpr int LPR = 4;
+    const int rhs = blockIdx.y;
+    const cx<R> *x = xall + rhs * vs;
+    cx<R> *y = yall + rhs * vs;
+    bool skip = false;
+    if (EPI == EPI_RESID && mask && !mask[rhs]) return;
+    if (EPI == EPI_DOT) {
+        if (scal[rhs].done) return;
+        skip = scal[rhs].conv != 0;
+    }
+    double acc[2] = {0.0, 0.0};
+    const int lane = threadIdx.x % LPR;
+    if (!skip) {
+        for (int64_t i = (blockIdx.x * (int64_t)BS + threadIdx.x) / LPR; i < m; i += (int64_t)gridDim.x * (BS / LPR)) {
+            c64 s = mk<double>(0, 0);
+            for (int64_t k = rowptr[i] + lane; k < rowptr[i + 1]; k += LPR) {
+                const int32_t j = colind[k];
+                c64 xv = cvt<double>(x[j]);
+                if (MODE == MODE_P) xv = xv * qin[j];
+                s = s + val[k] * xv;
+            }
+#pragma unroll
+            for (int o = 1; o < LPR; o <<= 1) {
+                s.re += __shfl_xor(s.re, o, LPR);
+                s.im += __shfl_xor(s.im, o, LPR);
+            }
+            if (lane) continue;
+            const c64 xi = cvt<double>(x[i]);
+            const c64 out = MODE == MODE_P ? qout[i] * s : s;
+            if (EPI == EPI_RESID) {
+                const c64 bb = ball[rhs * vs + i];
+                const c64 rr = mk<double>(bb.re - out.re, bb.im - out.im);
+                y[i] = cvt<R>(rr);
+                acc[0] += rr.re * rr.re + rr.im * rr.im;
+            } else {
+                y[i] = cvt<R>(out);
+                if (EPI == EPI_DOT) {
+                    acc[0] += xi.re * out.re - xi.im * out.im;
+                    acc[1] += xi.re * out.im + xi.im * out.re;
+                }
+            }
+        }
+    }
+    if (EPI == EPI_STORE) return;
+    block_partial<2>(acc, part + (size_t)rhs * NPART * 4, (int)blockIdx.x);
+}
+
+// Jacobi factor q = diag(A)^-1/2 of a row-major A; flag set for a zero or
+// missing diagonal entry
+__global__ void k_csr_jacobi(int64_t m, const int64_t *rowptr, const int32_t *colind, const c64 *val, c64 *q,
+                             int *flag) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    c64 d = mk<double>(0, 0);
+    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+        if (colind[k] == i) d = val[k];
+    if (iszero(d)) { atomicOr(flag, 1); d = mk<double>(1, 0); }
+    q[i] = cinv(csqrt_(d));
+}
+
+// Symmetrizing scaling of an assembled A: a diagonal S with S A symmetric,
+// s_i A_ij = s_j A_ji, exists for the A of create_A (S = the E-cell volumes,
+// since Cₘ = S⁻¹ Cₑᵀ S_H) but A alone does not carry it.  It is spread
+// breadth-first from one DOF (s = 1): each sweep gives every unscaled DOF i
+// the scale of its first scaled neighbour j (ascending j, A_ij and A_ji both
+// nonzero), s_i = s_j A_ji / A_ij, reading only the previous sweep's scales
+// (s = 0: not yet scaled).
+__global__ void k_sym_sweep(int64_t n, const int64_t *rowptr, const int32_t *colind, const c64 *val,
+                            const int64_t *colptr, const int64_t *rowval, const c64 *nzval, const c64 *s_old,
+                            c64 *s_new, int *changed) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const c64 si = s_old[i];
+    if (!iszero(si)) { s_new[i] = si; return; }
+    c64 v = mk<double>(0, 0);
+    // row i (A_ij, columns ascending) against column i (A_ji, rows ascending)
+    int64_t a = rowptr[i], ae = rowptr[i + 1], b = colptr[i] - 1, be = colptr[i + 1] - 1;
+    while (a < ae && b < be) {
+        const int64_t ja = colind[a], jb = rowval[b] - 1;
+        if (ja == jb) {
+            const c64 sj = s_old[ja];
+            if (!iszero(sj) && !iszero(val[a]) && !iszero(nzval[b])) { v = sj * cdiv(nzval[b], val[a]); break; }
+            ++a;
+            ++b;
+        } else if (ja < jb) ++a;
+        else ++b;
+    }
+    s_new[i] = v;
+    if (!iszero(v)) *changed = 1;
+}
+// first DOF still unscaled (a new component) gets s = 1
+__global__ void k_sym_seed(int64_t n, c64 *s, unsigned long long *first) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n && iszero(s[i])) atomicMin(first, (unsigned long long)i);
+}
+__global__ void k_sym_set1(c64 *s, const unsigned long long *first, int64_t n) {
+    if (threadIdx.x == 0 && *first < (unsigned long long)n) s[*first] = mk<double>(1, 0);
+}
+// flag if some stored pair misses |s_i A_ij − s_j A_ji| ≤ 1e-10 (|s_i A_ij| + |s_j A_ji|)
+__global__ void k_sym_check(int64_t n, const int64_t *rowptr, const int32_t *colind, const c64 *val,
+                            const int64_t *colptr, const int64_t *rowval, const c64 *nzval, const c64 *sv, int *flag) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t a = rowptr[i], ae = rowptr[i + 1], b = colptr[i] - 1, be = colptr[i + 1] - 1;
+    while (a < ae || b < be) {
+        const int64_t ja = a < ae ? colind[a] : INT64_MAX, jb = b < be ? rowval[b] - 1 : INT64_MAX;
+        const int64_t j = ja < jb ? ja : jb;
+        const c64 aij = ja == j ? val[a] : mk<double>(0, 0), aji = jb == j ? nzval[b] : mk<double>(0, 0);
+        if (ja == j) ++a;
+        if (jb == j) ++b;
+        const c64 l = sv[i] * aij, r = sv[j] * aji;
+        const double d = hypot(l.re - r.re, l.im - r.im), m = hypot(l.re, l.im) + hypot(r.re, r.im);
+        if (!(d <= 1e-10 * m)) { atomicOr(flag, 1); return; }
+    }
+}
+// T = √s and the solver's scalings: qs = q T⁻¹ (input), qo = q T (output)
+__global__ void k_sym_scale(int64_t n, const c64 *sv, const c64 *q, c64 *t, c64 *tinv, c64 *qs, c64 *qo) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const c64 ti = csqrt_(sv[i]), tv = cinv(ti);
+    t[i] = ti;
+    tinv[i] = tv;
+    qs[i] = q[i] * tv;
+    qo[i] = q[i] * ti;
+}
+
 #include "stencil_tiled.hpp"
 
 // ------------------------------------------------------- vector kernels
@@ -553,7 +695,9 @@ __global__ __launch_bounds__(BS) void k_fin_r(const double *part, int nb, Scal *
 }
 
 // d = c²θ² d + c²α q;  z += d;  q = r + β q
-template <typename R, bool NT>
+// QNT: the new q is streamed too (the q vectors of all RHS exceed the cache,
+// so keeping them there only evicts other data)
+template <typename R, bool NT, bool QNT = false>
 __global__ __launch_bounds__(BS) void k_upd_dzq(cx<R> *__restrict__ dall, cx<R> *__restrict__ zall,
                                                 cx<R> *__restrict__ qall, const cx<R> *__restrict__ rall, int64_t n,
                                                 int64_t vs, const Scal *scal) {
@@ -569,7 +713,9 @@ __global__ __launch_bounds__(BS) void k_upd_dzq(cx<R> *__restrict__ dall, cx<R> 
         c64 de = c2t2 * cvt<double>(vld<NT>(&d[e])) + c2a * qe;
         vst<NT>(&d[e], cvt<R>(de));
         vst<NT>(&z[e], cvt<R>(cvt<double>(vld<NT>(&z[e])) + de));
-        q[e] = cvt<R>(cvt<double>(vld<NT>(&r[e])) + be * qe);   // read next by the stencil: cached
+        const cx<R> qn = cvt<R>(cvt<double>(vld<NT>(&r[e])) + be * qe);   // read next by the stencil
+        if (QNT) vst<true>(&q[e], qn);
+        else q[e] = qn;
     }
 }
 
@@ -869,6 +1015,7 @@ void Workspace::release() {
 // scratch of the two-pass direct kernel for nrhs right-hand sides (operators
 // the tiled stencils do not serve); allocated outside any graph capture
 static int hbuf_ensure(mfd_op *op, int64_t nrhs) {
+    if (op->csr) return MFD_OK;
     const bool tiled_ok = !op->has_mu && op->ee_all;
     if (tiled_ok && op->tile64[0] > 0 && op->tile32[0] > 0) return MFD_OK;
     const size_t need = sizeof(c64) * 3 * (size_t)op->G.h.M * (size_t)nrhs;
@@ -972,6 +1119,14 @@ static int launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int6
     int ty = std::is_same<R, double>::value ? op->tile64[0] : op->tile32[0];
     int kz = std::is_same<R, double>::value ? op->tile64[1] : op->tile32[1];
     int xr = std::is_same<R, double>::value ? op->tile64[2] : op->tile32[2];
+    if (op->csr) {   // an operator built from a caller's CSC matrix
+        dim3 grid(ngrid(vs), (unsigned)nrhs);
+        k_spmv<R, MODE, EPI><<<grid, BS, 0, s>>>(vs, op->csr_rowptr, op->csr_colind, op->csr_val, op->d64.qs,
+                                                 op->csr_qo, x, y, b, vs, part, scal, mask);
+        if (after_main) hipEventRecord(after_main, s);
+        if (finish) launch_finish<EPI>(s, (int)grid.x, nrhs, part, scal, rn, mask);
+        return (int)grid.x;
+    }
     if (MODE == MODE_P && op->gd && ty == 8) ty = 4;   // the grad-div tile keeps two more LDS planes
     if (!op->has_mu && op->ee_all && ty > 0) {
         int nx = (g.N[0] + 63) / 64, ny = (g.N[1] + ty - 1) / ty;
@@ -1027,7 +1182,11 @@ static int launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int6
 template <typename R>
 static void launch_upd_dzq(dim3 gv, hipStream_t s, cx<R> *d, cx<R> *z, cx<R> *q, const cx<R> *r, int64_t n,
                            const Scal *scal, bool nt) {
-    if (nt) k_upd_dzq<R, true><<<gv, BS, 0, s>>>(d, z, q, r, n, n, scal);
+    // the new q of every RHS fits the 256 MB Infinity Cache: keep it there for
+    // the next stencil; otherwise stream it as well
+    const bool qnt = nt && (double)n * gv.y * sizeof(cx<R>) > 256.0 * 1024 * 1024;
+    if (qnt) k_upd_dzq<R, true, true><<<gv, BS, 0, s>>>(d, z, q, r, n, n, scal);
+    else if (nt) k_upd_dzq<R, true><<<gv, BS, 0, s>>>(d, z, q, r, n, n, scal);
     else k_upd_dzq<R, false><<<gv, BS, 0, s>>>(d, z, q, r, n, n, scal);
 }
 
@@ -1049,7 +1208,11 @@ static void launch_iter(mfd_op *op, int64_t nrhs, hipEvent_t *E = nullptr) {
     Workspace &w = op->ws;
     int64_t n = w.n;
     hipStream_t s = op->stream;
-    dim3 gv(ngrid(n), (unsigned)nrhs);
+    // about one resident round of blocks in total: every RHS streams at once
+    // (with ngrid(n) blocks per RHS the launch ran as nrhs successive rounds,
+    // measured 24% slower per RHS for the FP64 d/z/q update at 4 RHS)
+    const unsigned gx = std::max<unsigned>(ngrid(n) / (unsigned)nrhs, std::min<unsigned>(ngrid(n), 64u));
+    dim3 gv(gx, (unsigned)nrhs);
     size_t pslot = (size_t)nrhs * NPART * 4;
     const bool nt = stream_nt(n, nrhs, sizeof(cx<R>));
     if (E) hipEventRecord(E[0], s);
@@ -1205,9 +1368,123 @@ int op_destroy_impl(mfd_op *op) {
     op->ws.release();
     if (op->buf) hipFree(op->buf);
     if (op->hbuf) hipFree(op->hbuf);
+    if (op->csr_buf) hipFree(op->csr_buf);
     if (op->ev0) hipEventDestroy(op->ev0);
     if (op->ev1) hipEventDestroy(op->ev1);
     delete op;
+    return MFD_OK;
+}
+
+// Operator from the caller's assembled A (the `A` of create_A/create_linsys,
+// model.jl:209-246): a row-major device copy applied by k_spmv, Jacobi
+// scaling from its diagonal, and a 1D unit grid of n/3 cells for the solver's
+// DOF numbering (so T = 1 and P = Q A Q).  A that equals its transpose is
+// complex-symmetric (SQMR, COCG, adjoint); otherwise BiCGSTAB serves it.
+int op_create_csc_impl(mfd_ctx *ctx, const mfd_csc *A, mfd_op **out) {
+    if (!A || A->m != A->n || A->n < 3 || A->n % 3) {
+        set_error("mfd_op_create_csc: A must be square with n = 3·(cells) rows");
+        return MFD_ERR_ARG;
+    }
+    const int64_t n = A->n, M = n / 3;
+    std::vector<double> l0(M + 1), l1 = {0.0, 1.0};
+    for (int64_t i = 0; i <= M; ++i) l0[i] = (double)i;
+    mfd_grid g;
+    memset(&g, 0, sizeof g);
+    g.N[0] = M;
+    g.N[1] = g.N[2] = 1;
+    g.lprim[0] = l0.data();
+    g.lprim[1] = g.lprim[2] = l1.data();
+    g.omega_pml.re = 1.0;
+    g.pml_m = 4.0;
+    g.pml_lnR = -16.0;
+    g.order_cmpfirst = 1;
+    DevCsc dA;
+    MFD_TRY(csc_upload(ctx, A, "A", dA));
+    DevCsr R;
+    MFD_TRY(csr_of(ctx, dA, R));
+    mfd_op *op = new mfd_op();
+    op->ctx = ctx;
+    op->stream = ctx->stream;
+    op->ft = MFD_EE;
+    auto bail = [&](int code) { op_destroy_impl(op); return code; };
+    int rc = op->G.create(&g, ctx->stream);
+    if (rc) return bail(rc);
+    hipStream_t s = ctx->stream;
+    op->csr = 1;
+    op->csr_buf = R.buf;
+    op->csr_rowptr = R.rowptr;
+    op->csr_colind = R.colind;
+    op->csr_val = R.val;
+    R.buf = nullptr;   // the operator owns it now
+    op->tile64[0] = op->tile32[0] = 0;
+    op->omega = mk<double>(0, 0);
+    const size_t b64 = sizeof(c64) * n, b32 = sizeof(c32) * n;
+    // q, qs, qo, T, T⁻¹ (FP64), q (FP32), flags
+    if (hipMalloc(&op->buf, 5 * b64 + b32 + 256) != hipSuccess) { set_error("hipMalloc failed for operator arrays"); return bail(MFD_ERR_ALLOC); }
+    c64 *q = (c64 *)op->buf, *qs = q + n, *qo = q + 2 * n, *t = q + 3 * n, *tinv = q + 4 * n;
+    c32 *q32 = (c32 *)(q + 5 * n);
+    int *flag = (int *)((char *)op->buf + 5 * b64 + b32 + 64);
+    int fl[2] = {0, 0};
+    if (hipMemsetAsync(flag, 0, 2 * sizeof(int), s) != hipSuccess) return bail(MFD_ERR_HIP);
+    k_csr_jacobi<<<nb(n), BS, 0, s>>>(n, op->csr_rowptr, op->csr_colind, op->csr_val, q, flag);
+    // symmetrizing scaling: breadth-first sweeps (checked every 16), a new
+    // seed for each component, then the check
+    void *tmp = nullptr;
+    if (hipMalloc(&tmp, 2 * sizeof(c64) * n + 256) != hipSuccess) { set_error("hipMalloc failed"); return bail(MFD_ERR_ALLOC); }
+    c64 *w = (c64 *)tmp, *w2 = w + n;
+    int *chg = (int *)(w2 + n);
+    unsigned long long *first = (unsigned long long *)(chg + 32);
+    hipError_t se = hipMemsetAsync(w, 0, sizeof(c64) * n, s);
+    int64_t sweeps = 0;
+    while (se == hipSuccess) {
+        // seed the first unscaled DOF (DOF 0 at the start)
+        const unsigned long long none = ~0ull;
+        hipMemcpyAsync(first, &none, sizeof none, hipMemcpyHostToDevice, s);
+        k_sym_seed<<<nb(n), BS, 0, s>>>(n, w, first);
+        unsigned long long f = none;
+        hipMemcpyAsync(&f, first, sizeof f, hipMemcpyDeviceToHost, s);
+        se = hipStreamSynchronize(s);
+        if (se != hipSuccess || f == none) break;   // every DOF scaled
+        k_sym_set1<<<1, 64, 0, s>>>(w, first, n);
+        while (sweeps <= n) {
+            hipMemsetAsync(chg, 0, 16 * sizeof(int), s);
+            for (int k = 0; k < 16; ++k) {
+                k_sym_sweep<<<nb(n), BS, 0, s>>>(n, op->csr_rowptr, op->csr_colind, op->csr_val, dA.colptr, dA.rowval,
+                                                  dA.nzval, w, w2, chg + k);
+                std::swap(w, w2);
+            }
+            sweeps += 16;
+            int last = 1;
+            hipMemcpyAsync(&last, chg + 15, sizeof(int), hipMemcpyDeviceToHost, s);
+            se = hipStreamSynchronize(s);
+            if (se != hipSuccess || last == 0) break;   // the component is covered
+        }
+    }
+    k_sym_check<<<nb(n), BS, 0, s>>>(n, op->csr_rowptr, op->csr_colind, op->csr_val, dA.colptr, dA.rowval, dA.nzval, w,
+                                     flag + 1);
+    k_sym_scale<<<nb(n), BS, 0, s>>>(n, w, q, t, tinv, qs, qo);
+    k_down<<<nb(n), BS, 0, s>>>(q, q32, n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(fl, flag, 2 * sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(tmp);
+    if (e != hipSuccess) { set_error("operator set-up failed"); return bail(MFD_ERR_HIP); }
+    if (fl[0]) { set_error("mfd_op_create_csc: A has a zero or missing diagonal entry (Jacobi scaling undefined)"); return bail(MFD_ERR_ARG); }
+    op->sym = fl[1] == 0;
+    if (op->sym) {   // P = Q T A T⁻¹ Q is complex-symmetric: SQMR, COCG and the adjoint apply
+        op->G.d64.tdof = t;
+        op->G.d64.tdofinv = tinv;
+        op->csr_qo = qo;
+        op->d64.qs = qs;
+    } else {         // T = 1 (BiCGSTAB)
+        op->csr_qo = q;
+        op->d64.qs = q;
+    }
+    op->d64.q = q;
+    op->d32.q = q32;
+    hipEventCreate(&op->ev0);
+    hipEventCreate(&op->ev1);
+    *out = op;
     return MFD_OK;
 }
 
@@ -1359,15 +1636,22 @@ int op_bench_iteration_impl(mfd_op *op, int64_t nrhs, int prec, int iters, doubl
 // operator P on FP64 or complex-FP32 vectors) for each candidate (TY, KZ, tile
 // order) and keeps the fastest per precision.
 int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best32, double *ms_out) {
+    if (op->csr) { set_error("autotune: an operator built from a CSC matrix has no stencil tiles"); return MFD_ERR_UNSUPPORTED; }
     if (op->has_mu || !op->ee_all) {   // the direct kernel serves these operators
         op->tile64[0] = op->tile32[0] = 0;
         if (best64) best64[0] = best64[1] = best64[2] = 0;
         if (best32) best32[0] = best32[1] = best32[2] = 0;
         return MFD_OK;
     }
+    // Each candidate is timed in context: complete SQMR steps (stencil, r
+    // update, finish, d/z/q update) on the solver workspace with nonzero data,
+    // scored by the whole step, so the caches hold what they hold inside a
+    // solve (a tile order whose halo rows miss L2 costs the stencil and, through
+    // the caches, the vector kernels around it).
     const int tys[4] = {1, 2, 4, 8}, kzs[4] = {4, 8, 16, 32};
     double bestms[2] = {1e30, 1e30};
     int bt[2][3] = {{4, 16, 1}, {4, 16, 1}};
+    const int its = std::max(2, std::min(iters, 50));
     for (int prec = 0; prec < 2; ++prec)
         for (int a = 0; a < 4; ++a)
             for (int b = 0; b < 4; ++b)
@@ -1376,8 +1660,9 @@ int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best
                     tile[0] = tys[a];
                     tile[1] = kzs[b];
                     tile[2] = xo;
-                    double ms = 0;
-                    MFD_TRY(time_apply(op, nrhs, prec == 0 ? MFD_FP64 : MFD_MIXED, MODE_P, iters, &ms));
+                    double part[3] = {0, 0, 0};
+                    MFD_TRY(op_bench_iteration_impl(op, nrhs, prec == 0 ? MFD_FP64 : MFD_MIXED, its, part));
+                    const double ms = part[0] + part[1] + part[2];
                     if (ms < bestms[prec]) {
                         bestms[prec] = ms;
                         bt[prec][0] = tys[a];
@@ -1567,7 +1852,7 @@ static void launch_cocg_iter(mfd_op *op, int64_t nrhs, int par) {
     cx<R> *po = cocg_p<R>(w, par), *pn = cocg_p<R>(w, par ^ 1);
     const DGrid<R> &g = op->G.view<R>();
     int ty = std::is_same<R, double>::value ? op->tile64[0] : op->tile32[0];
-    if (!op->full && !op->has_mu && !op->gd && op->ee_all && ty > 0) {
+    if (!op->csr && !op->full && !op->has_mu && !op->gd && op->ee_all && ty > 0) {
         int kz = std::is_same<R, double>::value ? op->tile64[1] : op->tile32[1];
         int xr = std::is_same<R, double>::value ? op->tile64[2] : op->tile32[2];
         int nx = (g.N[0] + 63) / 64, ny = (g.N[1] + ty - 1) / ty, nch = (g.N[2] + kz - 1) / kz;
@@ -1773,6 +2058,7 @@ static int run_cocg(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_
 }
 
 int op_set_tile_impl(mfd_op *op, int prec, int ty, int kz, int xcd) {
+    if (op->csr) { set_error("tile: an operator built from a CSC matrix has no stencil tiles"); return MFD_ERR_UNSUPPORTED; }
     if (!(ty == 0 || ty == 1 || ty == 2 || ty == 4 || ty == 8) || kz < 1 || (xcd != 0 && xcd != 1)) {
         set_error("tile: TY in {0,1,2,4,8}, KZ >= 1, order 0 or 1");
         return MFD_ERR_ARG;
@@ -2755,6 +3041,10 @@ __global__ __launch_bounds__(BS) void k_e_from_h(DGrid<double> g, OpData<double>
 
 // which = 0: out = h_from_e(in, src = jₘ); which = 1: out = e_from_h(in, src = jₑ)
 int field_convert_impl(mfd_op *op, int which, const mfd_c64 *in, const mfd_c64 *src, mfd_c64 *out) {
+    if (op->csr) {
+        set_error("h_from_e/e_from_h: the operator holds only A; use mfd_h_from_e_csc / mfd_e_from_h_csc with Pμ, Cₑ / Pε, Cₘ");
+        return MFD_ERR_UNSUPPORTED;
+    }
     if (op->ft != MFD_EE) {
         set_error("h_from_e/e_from_h: operator of the EE formulation");
         return MFD_ERR_UNSUPPORTED;
@@ -2792,6 +3082,7 @@ int field_convert_impl(mfd_op *op, int which, const mfd_c64 *in, const mfd_c64 *
 }
 
 int adjoint_grad_impl(mfd_op *op, const mfd_c64 *e, const mfd_c64 *lam, double *grad) {
+    if (op->csr) { set_error("adjoint_grad: needs an operator built from a model (ω and the grid)"); return MFD_ERR_UNSUPPORTED; }
     int64_t n = 3 * op->G.h.M;
     void *d = nullptr;
     MFD_HIP(hipMalloc(&d, (2 * sizeof(c64) + sizeof(double)) * n));

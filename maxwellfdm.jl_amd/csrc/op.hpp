@@ -87,4 +87,13 @@ struct mfd_op {
     void *hbuf = nullptr;     // H = μ⁻¹ Cₑ x scratch of the two-pass direct kernel (3M values per RHS)
     size_t hbuf_bytes = 0;
     int ee_all = 1;           // boundft = EE on every axis (the tiled stencils and field conversions need it)
+    // an operator built from a caller's CSC matrix A (mfd_op_create_csc):
+    // row-major copy of A, applied by a sparse product; its grid is a 1D unit
+    // grid of n/3 cells that only provides the DOF numbering (T = 1)
+    int csr = 0;
+    void *csr_buf = nullptr;
+    int64_t *csr_rowptr = nullptr;
+    int32_t *csr_colind = nullptr;
+    mfd::c64 *csr_val = nullptr;
+    const mfd::c64 *csr_qo = nullptr;   // output scaling Q T of the sparse product
 };

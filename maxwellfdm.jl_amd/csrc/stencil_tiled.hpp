@@ -145,6 +145,10 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
     // (dispatch index L goes to XCD L mod 8); with xcd != 0 each XCD gets one
     // contiguous run of tiles in x-then-y order, so the y halo rows a tile
     // stages are read by its neighbour on the same XCD and hit that XCD's L2.
+    // The right-hand side is the fastest-varying index of that run: the blocks
+    // of one tile for every RHS are neighbours in it, so they run together on
+    // one XCD and all but the first find the tile's operator data (qs, corner
+    // ε) and its halo rows in that XCD's L2.
     const unsigned nbx = gridDim.x, nby = gridDim.y;
     const unsigned T = nbx * nby * gridDim.z;
     unsigned L = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z), lt = L;
@@ -152,8 +156,9 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
         const unsigned q = T / 8, rem = T % 8, x = L % 8;
         lt = x * q + (x < rem ? x : rem) + L / 8;
     }
-    const int bx = (int)(lt % nbx), by = (int)((lt / nbx) % nby), bz = (int)(lt / (nbx * nby));
-    const int rhs = bz / nch, chunk = bz - rhs * nch;
+    const unsigned nrhs_l = gridDim.z / (unsigned)nch, tile = lt / nrhs_l;
+    const int rhs = (int)(lt - tile * nrhs_l);
+    const int bx = (int)(tile % nbx), by = (int)((tile / nbx) % nby), chunk = (int)(tile / (nbx * nby));
     const cx<R> *x = xall + rhs * vs;
     cx<R> *y = yall + rhs * vs;
     const cx<R> *rr = FUSE ? rall + rhs * vs : nullptr;

@@ -11,7 +11,7 @@ from .core import (DUAL, EE, HH, PRIM, Ball, Box, Cuboid, Grid, MaxwellOperator,
                    create_A, create_b, create_curls, create_Mcs, Mcs_csc, create_eikL, create_field_array, create_linsys,
                    create_param_array, create_paramops, create_srcs, create_stretched_dls, curl_csc,
                    distweights, field_arr2vec, nX, nY, nZ, paramop_csc, set_boundft, set_kbloch, set_Npml,
-                   set_omegapml, set_ωpml, solve_batch, to_scipy)
+                   set_omegapml, set_ωpml, solve_batch, solve_linsys, to_scipy, h_from_e, e_from_h)
 
 from . import twod
 from . import oned

@@ -28,6 +28,12 @@ class mfd_grid(C.Structure):
                 ("order_cmpfirst", C.c_int32)]
 
 
+class mfd_csc(C.Structure):
+    """A Julia SparseMatrixCSC{ComplexF64,Int64} as it lies in memory (1-based)."""
+    _fields_ = [("m", C.c_int64), ("n", C.c_int64), ("nnz", C.c_int64), ("colptr", C.c_void_p),
+                ("rowval", C.c_void_p), ("nzval", C.c_void_p)]
+
+
 class mfd_shape(C.Structure):
     _fields_ = [("kind", C.c_int32), ("pad", C.c_int32), ("c", C.c_double * 3), ("r", C.c_double),
                 ("s", C.c_double * 3)]
@@ -89,6 +95,12 @@ SIGNATURES = {
     "mfd_A_csc": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int, c64, _P, _P, _P, _P, _P]),
     "mfd_create_b": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int, c64, _P, _P, _P, _P, _P]),
     "mfd_op_create": (C.c_int, [_P, C.POINTER(mfd_grid), C.c_int, c64, _P, _P, C.POINTER(_P)]),
+    "mfd_op_create_csc": (C.c_int, [_P, C.POINTER(mfd_csc), C.POINTER(_P)]),
+    "mfd_A_from_csc_nnz": (C.c_int, [_P, C.c_int, c64] + [C.POINTER(mfd_csc)] * 4 + [_I64P]),
+    "mfd_A_from_csc": (C.c_int, [_P, C.c_int, c64] + [C.POINTER(mfd_csc)] * 4 + [_P, _P, _P]),
+    "mfd_b_from_csc": (C.c_int, [_P, C.c_int, c64] + [C.POINTER(mfd_csc)] * 4 + [_P, _P, _P]),
+    "mfd_h_from_e_csc": (C.c_int, [_P, _P, c64, C.POINTER(mfd_csc), C.POINTER(mfd_csc), _P, _P]),
+    "mfd_e_from_h_csc": (C.c_int, [_P, _P, c64, C.POINTER(mfd_csc), C.POINTER(mfd_csc), _P, _P]),
     "mfd_op_destroy": (C.c_int, [_P]),
     "mfd_op_apply": (C.c_int, [_P, _P, _P, C.c_int64]),
     "mfd_op_apply_dev": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int]),

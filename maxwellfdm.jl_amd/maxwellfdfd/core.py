@@ -405,20 +405,115 @@ def create_Mcs(mdl):
     return to_scipy(Mcs_csc(mdl, 0)), to_scipy(Mcs_csc(mdl, 1))
 
 
-def create_A(ft, ω, mdl, εarr=None, μarr=None):
-    """create_A (model.jl:225-246) assembled on the device in closed form.
+class _CscArg:
+    """A sparse matrix as the Julia SparseMatrixCSC{ComplexF64,Int64} layout
+    (1-based colptr/rowval, rows ascending, explicit zeros kept) handed to the
+    C ABI; holds the arrays while the call runs."""
 
-    The reference takes the assembled (Pε, Pμ, Cₑ, Cₘ); the device path
-    assembles the same A directly from the model's grid and parameter arrays.
+    def __init__(self, A):
+        import scipy.sparse as sp
+        A = sp.csc_matrix(A, dtype=np.complex128, copy=True)
+        A.sum_duplicates()
+        A.sort_indices()
+        self.colptr = (A.indptr.astype(np.int64) + 1)
+        self.rowval = (A.indices.astype(np.int64) + 1)
+        self.nzval = np.ascontiguousarray(A.data, dtype=np.complex128)
+        self.s = L.mfd_csc(A.shape[0], A.shape[1], A.nnz, self.colptr.ctypes.data,
+                           self.rowval.ctypes.data if A.nnz else None, self.nzval.ctypes.data if A.nnz else None)
+
+    def ref(self):
+        return C.byref(self.s)
+
+
+def _is_model(x):
+    return isinstance(x, Model)
+
+
+def _unpack(args, k):
+    """The reference's tuple forms: create_A(ft, ω, (Pε, Pμ), (Cₑ, Cₘ)) etc."""
+    out = []
+    for a in args:
+        out.extend(a if isinstance(a, tuple) else (a,))
+    if len(out) != k:
+        raise MfdArgumentError(L.MFD_ERR_ARG, f"expected {k} matrices/vectors, got {len(out)}")
+    return out
+
+
+def create_A(ft, ω, *args, εarr=None, μarr=None):
+    """create_A (model.jl:222-246).
+
+    create_A(ft, ω, Pε, Pμ, Cₑ, Cₘ) or create_A(ft, ω, (Pε, Pμ), (Cₑ, Cₘ)):
+    the reference's signature; A = Cₘ (Pμ \\ Cₑ) − ω² Pε (EE) or
+    Cₑ (Pε \\ Cₘ) − ω² Pμ (HH) formed on the device from the given matrices.
+    create_A(ft, ω, mdl): the same A assembled on the device in closed form
+    from the model's grid and parameter arrays (no matrices needed).
     """
-    εarr = mdl.εarr if εarr is None else εarr
-    μarr = mdl.μarr if μarr is None else μarr
-    return to_scipy(A_csc(mdl, ω, εarr, μarr, ft))
+    if len(args) == 1 and _is_model(args[0]):
+        mdl = args[0]
+        εarr = mdl.εarr if εarr is None else εarr
+        μarr = mdl.μarr if μarr is None else μarr
+        return to_scipy(A_csc(mdl, ω, εarr, μarr, ft))
+    Pe, Pm, Ce, Cm = (_CscArg(a) for a in _unpack(args, 4))
+    n = Pe.s.n
+    return to_scipy(_csc_call(lib().mfd_A_from_csc_nnz, lib().mfd_A_from_csc,
+                              (context(), ft, L.cplx(ω), Pe.ref(), Pm.ref(), Ce.ref(), Cm.ref()), n))
 
 
-def create_b(ft, ω, mdl, jₑ, jₘ, μarr=None, εarr=None):
-    """create_b (model.jl:251-274) on the device: b = −Cₘ (Pμ \\ jₘ) − iω jₑ
-    (ft = EE) or b = Cₑ (Pε \\ jₑ) − iω jₘ (ft = HH)."""
+def create_b(ft, ω, *args, μarr=None, εarr=None):
+    """create_b (model.jl:248-274): b = −Cₘ (Pμ \\ jₘ) − iω jₑ (ft = EE) or
+    b = Cₑ (Pε \\ jₑ) − iω jₘ (ft = HH), on the device.
+
+    create_b(ft, ω, Pε, Pμ, Cₑ, Cₘ, jₑ, jₘ) or create_b(ft, ω, (Pε, Pμ),
+    (Cₑ, Cₘ), (jₑ, jₘ)): from the given matrices, as the reference.
+    create_b(ft, ω, mdl, jₑ, jₘ): from the model's grid and parameter arrays.
+    """
+    if _is_model(args[0]):
+        return _create_b_model(ft, ω, *args, μarr=μarr, εarr=εarr)
+    Pe, Pm, Ce, Cm, je, jm = _unpack(args, 6)
+    je = L.carr(je)
+    jm = L.carr(jm)
+    mats = [_CscArg(a) for a in (Pe, Pm, Ce, Cm)]
+    if je.size != mats[0].s.n or jm.size != mats[0].s.n:
+        raise MfdArgumentError(L.MFD_ERR_ARG, "create_b: jₑ and jₘ must have n entries")
+    b = np.zeros_like(je)
+    check(lib().mfd_b_from_csc(context(), ft, L.cplx(ω), *(m.ref() for m in mats), ptr(je), ptr(jm), ptr(b)))
+    return b
+
+
+def _field_csc(fn, v, ω, P, Cx, j):
+    v = L.carr(v)
+    Pa, Ca = _CscArg(P), _CscArg(Cx)
+    if v.size != Ca.s.n:
+        raise MfdArgumentError(L.MFD_ERR_ARG, "field vector must have n entries")
+    j = None if j is None else L.carr(j)
+    if j is not None and j.size != v.size:
+        raise MfdArgumentError(L.MFD_ERR_ARG, "source vector must have n entries")
+    out = np.zeros_like(v)
+    check(fn(context(), ptr(v), L.cplx(ω), Pa.ref(), Ca.ref(), None if j is None else ptr(j), ptr(out)))
+    return out
+
+
+def h_from_e(e, ω, *args):
+    """h_from_e (model.jl:276-279): (i/ω) (Pμ \\ (Cₑ e + jₘ)) on the device.
+    h_from_e(e, ω, Pμ, Cₑ, jₘ) or h_from_e(e, ω, (Pε, Pμ), (Cₑ, Cₘ), (jₑ, jₘ))."""
+    if len(args) == 3 and all(isinstance(a, tuple) for a in args):
+        M, Cs, js = args
+        return _field_csc(lib().mfd_h_from_e_csc, e, ω, M[1], Cs[0], js[1])
+    Pm, Ce, jm = args
+    return _field_csc(lib().mfd_h_from_e_csc, e, ω, Pm, Ce, jm)
+
+
+def e_from_h(h, ω, *args):
+    """e_from_h (model.jl:281-284): (−i/ω) (Pε \\ (Cₘ h − jₑ)) on the device.
+    e_from_h(h, ω, Pε, Cₘ, jₑ) or e_from_h(h, ω, (Pε, Pμ), (Cₑ, Cₘ), (jₑ, jₘ))."""
+    if len(args) == 3 and all(isinstance(a, tuple) for a in args):
+        M, Cs, js = args
+        return _field_csc(lib().mfd_e_from_h_csc, h, ω, M[0], Cs[1], js[0])
+    Pe, Cm, je = args
+    return _field_csc(lib().mfd_e_from_h_csc, h, ω, Pe, Cm, je)
+
+
+def _create_b_model(ft, ω, mdl, jₑ, jₘ, μarr=None, εarr=None):
     μarr = mdl.μarr if μarr is None else μarr
     εarr = mdl.εarr if εarr is None else εarr
     g = mdl._desc()
@@ -431,11 +526,18 @@ def create_b(ft, ω, mdl, jₑ, jₘ, μarr=None, εarr=None):
     return b
 
 
-def create_linsys(ft, ω, mdl, jₑ=None, jₘ=None):
-    """create_linsys (model.jl:209-220): (A, b)."""
-    if jₑ is None:
-        jₑ, jₘ = create_srcs(mdl)
-    return create_A(ft, ω, mdl), create_b(ft, ω, mdl, jₑ, jₘ)
+def create_linsys(ft, ω, *args):
+    """create_linsys (model.jl:209-220): (A, b).
+
+    create_linsys(ft, ω, Pε, Pμ, Cₑ, Cₘ, jₑ, jₘ) or with the tuples (Ps, Cs,
+    js), as the reference; create_linsys(ft, ω, mdl[, jₑ, jₘ]) from a model.
+    """
+    if _is_model(args[0]):
+        mdl = args[0]
+        jₑ, jₘ = args[1:3] if len(args) >= 3 else create_srcs(mdl)
+        return create_A(ft, ω, mdl), create_b(ft, ω, mdl, jₑ, jₘ)
+    Pe, Pm, Ce, Cm, je, jm = _unpack(args, 6)
+    return create_A(ft, ω, Pe, Pm, Ce, Cm), create_b(ft, ω, Pe, Pm, Ce, Cm, je, jm)
 
 
 # ------------------------------------------------------- matrix-free solve
@@ -473,6 +575,19 @@ def solve_batch(ops, bs, tol=1e-8, precision="fp64", maxit=None, inner_tol=None,
     return es, st
 
 
+def solve_linsys(A, b, tol=1e-8, precision="fp64", **kw):
+    """The user's `A \\ b` on the device: A a MaxwellOperator (matrix-free) or
+    the sparse A of create_A / create_linsys (uploaded as a sparse operator).
+    Returns (e, stats)."""
+    if isinstance(A, MaxwellOperator):
+        return A.solve(b, tol=tol, precision=precision, **kw)
+    op = MaxwellOperator.from_csc(A)
+    try:
+        return op.solve(b, tol=tol, precision=precision, **kw)
+    finally:
+        op.close()
+
+
 class MaxwellOperator:
     """Device-resident A of create_A (model.jl:225-246) with its solver:
     ft = EE, A = Cₘ μ⁻¹ Cₑ − ω² Pε; ft = HH, A = Cₑ ε⁻¹ Cₘ − ω² Pμ (built as
@@ -494,6 +609,20 @@ class MaxwellOperator:
         h = C.c_void_p()
         check(lib().mfd_op_create(context(), C.byref(g), ft, L.cplx(ω), ptr(e), ptr(u), C.byref(h)))
         self._h = h
+
+    @classmethod
+    def from_csc(cls, A):
+        """Operator from an assembled sparse A (mfd_op_create_csc): applied as a
+        sparse product; SQMR when A equals its transpose, BiCGSTAB otherwise."""
+        a = _CscArg(A)
+        self = object.__new__(cls)
+        self.mdl = None
+        self.ω = None
+        self.n = a.s.n
+        h = C.c_void_p()
+        check(lib().mfd_op_create_csc(context(), a.ref(), C.byref(h)))
+        self._h = h
+        return self
 
     def _nrhs(self, v, what):
         """Number of stacked DOF vectors in v; raises unless v holds a positive whole number of them."""
