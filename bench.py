@@ -65,7 +65,9 @@ def parse():
     ap.add_argument("--voxels", type=int, default=16, help="adjoint: finite-difference voxels")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
                     help="batched-rhs, uneven-batch: solver precision (mixed: complex-FP32 inner cycles)")
-    ap.add_argument("--maxit", type=int, default=2000, help="large-aniso: solver steps")
+    ap.add_argument("--maxit", type=int, default=2000, help="large-aniso, batched-rhs --bounded: solver steps")
+    ap.add_argument("--bounded", action="store_true",
+                    help="batched-rhs: stop at --maxit steps, report the residual history and the projection")
     return ap.parse_args()
 
 
@@ -176,7 +178,10 @@ def kernel_table(op, cells, iters=50):
 def run_batched_rhs(a):
     """N³ random boxes (the 40-box generator) with nrhs seeded dipoles solved in
     one batched launch (FP64 or mixed), against one single-RHS solve of the
-    first dipole in the same precision."""
+    first dipole in the same precision; with --bounded, the batched solve stops
+    at --maxit steps and reports its residual history, the per-RHS step cost
+    against a single RHS (solver-iteration kernels in context) and the
+    projected time to 1e-8."""
     import numpy as np
     import torch
     import maxwellfdfd as M
@@ -191,6 +196,35 @@ def run_batched_rhs(a):
     b_d = torch.from_numpy(B).to(dev)
     e_d = torch.zeros_like(b_d)
     torch.cuda.synchronize()
+    if a.bounded:
+        it_batch = op.bench_iteration(a.nrhs, a.precision, 5)
+        op.autotune(1, 5)
+        it_one = op.bench_iteration(1, a.precision, 10)
+        op.autotune(a.nrhs, 5)
+        print(f"[batched-rhs] {N}^3, {a.nrhs} right-hand sides: step {sum(it_batch):.3f} ms "
+              f"({sum(it_batch) / a.nrhs:.4f} per RHS) vs {sum(it_one):.4f} ms for one", file=sys.stderr, flush=True)
+
+        def solve():
+            t0 = time.perf_counter()
+            st = op.solve_dev(b_d.data_ptr(), e_d.data_ptr(), a.nrhs, tol=1e-8, maxit=a.maxit,
+                              precision=a.precision, raise_on_noconv=False)
+            torch.cuda.synchronize()
+            return st, time.perf_counter() - t0
+
+        (st, t_solve), lines = traced(solve)
+        op.close()
+        hist = residual_history(lines)
+        hist.append((int(st.iterations), float(st.relres), "end"))
+        ms_step = 1e3 * t_solve / max(st.iterations, 1)
+        return {"workload": f"batched-rhs {N}^3 random boxes, {a.nrhs} dipoles, {a.precision} SQMR bounded at "
+                            f"{a.maxit} steps",
+                "iteration_kernels_ms_batch": [round(x, 4) for x in it_batch],
+                "iteration_ms_per_rhs_batch": round(sum(it_batch) / a.nrhs, 4),
+                "iteration_ms_single_rhs": round(sum(it_one), 4),
+                "solve_s": round(t_solve, 3), "steps": int(st.iterations), "steps_fp32": int(st.iterations_fp32),
+                "relres_reached": st.relres, "converged": bool(st.converged), "ms_per_step_all_rhs": round(ms_step, 3),
+                "residual_history": [[int(k), float(r), w] for k, r, w in hist],
+                "projection": project(hist, st.converged, ms_step)}
     print(f"[batched-rhs] {N}^3, {a.nrhs} right-hand sides: batched solve", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     st = op.solve_dev(b_d.data_ptr(), e_d.data_ptr(), a.nrhs, tol=1e-8, precision=a.precision)
@@ -268,23 +302,34 @@ def run_uneven_batch(a):
 def traced(fn):
     """Runs fn() with the solver's trace on (MFD_SOLVE_TRACE: one line per
     inner cycle and per true-residual probe, written by the library to fd 2)
-    and returns (fn's result, the trace lines)."""
-    import tempfile
+    and returns (fn's result, the trace lines).  fd 2 goes through a pipe
+    whose reader thread echoes every line as it arrives, so a long solve
+    keeps printing its progress."""
+    import threading
     os.environ["MFD_SOLVE_TRACE"] = "1"
     sys.stderr.flush()
     saved = os.dup(2)
-    with tempfile.TemporaryFile(mode="w+b") as f:
-        os.dup2(f.fileno(), 2)
-        try:
-            out = fn()
-        finally:
-            os.dup2(saved, 2)
-            os.close(saved)
-            del os.environ["MFD_SOLVE_TRACE"]
-        f.seek(0)
-        lines = f.read().decode("utf-8", "replace").splitlines()
-    for line in lines:
-        print(line, file=sys.stderr, flush=True)
+    rfd, wfd = os.pipe()
+    lines = []
+
+    def pump():
+        with os.fdopen(rfd, "rb") as r:
+            for raw in r:
+                os.write(saved, raw)
+                lines.append(raw.decode("utf-8", "replace").rstrip("\n"))
+
+    th = threading.Thread(target=pump, daemon=True)
+    th.start()
+    os.dup2(wfd, 2)
+    os.close(wfd)
+    try:
+        out = fn()
+    finally:
+        sys.stderr.flush()
+        os.dup2(saved, 2)   # closes the pipe's last write end: the reader ends
+        th.join(timeout=10)
+        os.close(saved)
+        del os.environ["MFD_SOLVE_TRACE"]
     return out, lines
 
 
@@ -305,13 +350,28 @@ def residual_history(lines):
     return hist
 
 
+def project(hist, converged, ms_step):
+    """Steps and seconds to relative residual 1e-8, extrapolated from the last
+    two points of a residual history (None if it converged or made no progress)."""
+    import math
+    pts = [(k, r) for k, r, _ in hist if r > 0]
+    if converged or len(pts) < 2:
+        return None
+    (k0, r0), (k1, r1) = pts[-2], pts[-1]
+    if not (r1 < r0 and k1 > k0):
+        return None
+    spd = (k1 - k0) / math.log10(r0 / r1)
+    steps_needed = k1 + spd * math.log10(r1 / 1e-8)
+    return {"steps_per_decade": round(spd, 1), "steps_to_1e-8": int(steps_needed),
+            "s_to_1e-8": round(steps_needed * ms_step * 1e-3, 1)}
+
+
 def run_large_aniso(a):
     """384×384×256 anisotropic case (200 smoothed spheres, full 3×3 ε, 8
     dipoles): assembly on the device, apply bandwidth of the full-tensor
     stencil, and a bounded batched solve (a.maxit steps, a.precision) with its
     FP64 residual history, the time per step and the time to 1e-8 projected
     from the history's last decade of progress."""
-    import math
     import numpy as np
     import torch
     import maxwellfdfd as M
@@ -351,16 +411,7 @@ def run_large_aniso(a):
     hist = residual_history(lines)
     hist.append((int(st.iterations), float(st.relres), "end"))
     ms_step = 1e3 * t_solve / max(st.iterations, 1)
-    proj = None
-    pts = [(k, r) for k, r, _ in hist if r > 0]
-    if not st.converged and len(pts) >= 2:
-        (k0, r0), (k1, r1) = pts[-2], pts[-1]
-        # steps per decade over the last stretch of the history
-        if r1 < r0 and k1 > k0:
-            spd = (k1 - k0) / math.log10(r0 / r1)
-            steps_needed = k1 + spd * math.log10(r1 / 1e-8)
-            proj = {"steps_per_decade": round(spd, 1), "steps_to_1e-8": int(steps_needed),
-                    "s_to_1e-8": round(steps_needed * ms_step * 1e-3, 1)}
+    proj = project(hist, st.converged, ms_step)
     return {"workload": f"large-aniso {m.grid.N[0]}x{m.grid.N[1]}x{m.grid.N[2]}, 200 smoothed spheres "
                         f"(full eps tensors: {full}), {a.nrhs} dipoles, {a.precision} SQMR bounded at {a.maxit} steps",
             "cells": int(cells), "matparams_s": round(t_mat, 3), "tiles": [list(tiles[0]), list(tiles[1])],

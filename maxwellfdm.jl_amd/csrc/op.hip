@@ -2144,6 +2144,12 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
         }
         *iters_out = mx;
         if (all || mx >= budget) break;
+        if (done_it % 512 == 0 && getenv("MFD_SOLVE_TRACE")) {   // progress of a long cycle
+            double worst = 0;
+            for (int64_t r = 0; r < nrhs; ++r)
+                if (!w.scal_h[r].conv && w.scal_h[r].cn > 0) worst = std::max(worst, w.scal_h[r].tau / w.scal_h[r].cn);
+            fprintf(stderr, "[mfd step] it %lld recurrence tau/cn %.3e\n", mx, worst);
+        }
         bool probe_now = false;
         if (sizeof(R) == 4) probe_now = b && e && done_it % probe == 0;
         else if (b && e && tol > 0 && cyc_res > 0 && done_it >= next64) {
