@@ -57,7 +57,8 @@ def parse():
                     help="fixed stencil tiles 'TY,KZ,order/TY,KZ,order' (FP64/FP32) instead of the autotuner, "
                          "so that separate hardware-counter passes launch the same kernels")
     ap.add_argument("--workload", default="random-boxes",
-                    choices=["random-boxes", "batched-rhs", "uneven-batch", "adjoint", "large-aniso", "next-rows"],
+                    choices=["random-boxes", "batched-rhs", "uneven-batch", "adjoint", "large-aniso", "next-rows",
+                             "mixed-fgmres"],
                     help="random-boxes: the metric line; the others time the batched-RHS, uneven-problem "
                          "and adjoint configurations")
     ap.add_argument("--nrhs", type=int, default=4, help="batched-rhs: right-hand sides")
@@ -66,6 +67,11 @@ def parse():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
                     help="batched-rhs, uneven-batch: solver precision (mixed: complex-FP32 inner cycles)")
     ap.add_argument("--maxit", type=int, default=2000, help="large-aniso, batched-rhs --bounded: solver steps")
+    ap.add_argument("--method", default=None, choices=["sqmr", "cocg", "bicgstab"],
+                    help="large-aniso: Krylov method (default: SQMR for a complex-symmetric operator)")
+    ap.add_argument("--inner-steps", default="1000,4000",
+                    help="mixed-fgmres: FP32 SQMR steps per preconditioner application (comma-separated)")
+    ap.add_argument("--restart", type=int, default=30, help="mixed-fgmres: FGMRES restart length")
     ap.add_argument("--bounded", action="store_true",
                     help="batched-rhs: stop at --maxit steps, report the residual history and the projection")
     return ap.parse_args()
@@ -402,7 +408,7 @@ def run_large_aniso(a):
     def solve():
         t0 = time.perf_counter()
         st = op.solve_dev(b_d.data_ptr(), e_d.data_ptr(), a.nrhs, tol=1e-8, maxit=a.maxit, precision=a.precision,
-                          raise_on_noconv=False)
+                          method=a.method, raise_on_noconv=False)
         torch.cuda.synchronize()
         return st, time.perf_counter() - t0
 
@@ -413,7 +419,8 @@ def run_large_aniso(a):
     ms_step = 1e3 * t_solve / max(st.iterations, 1)
     proj = project(hist, st.converged, ms_step)
     return {"workload": f"large-aniso {m.grid.N[0]}x{m.grid.N[1]}x{m.grid.N[2]}, 200 smoothed spheres "
-                        f"(full eps tensors: {full}), {a.nrhs} dipoles, {a.precision} SQMR bounded at {a.maxit} steps",
+                        f"(full eps tensors: {full}), {a.nrhs} dipoles, {a.precision} {(a.method or 'sqmr').upper()} "
+                        f"bounded at {a.maxit} steps",
             "cells": int(cells), "matparams_s": round(t_mat, 3), "tiles": [list(tiles[0]), list(tiles[1])],
             "apply_ms": round(ms_apply, 4),
             "apply_GBps": round(bpc * cells / (ms_apply * 1e-3) / 1e9, 1),
@@ -476,6 +483,109 @@ def run_adjoint(a):
             "forward_s": round(t_fwd, 3), "adjoint_and_grad_s": round(t_adj, 3),
             "fd": "central, h = 1e-4, perturbed fields e + A(eps +- h)^-1 (b - A(eps +- h) e) to 1e-12",
             "fd_voxels": a.voxels, "fd_s": round(t_fd, 1), "fd_rel_err_max": max(errs), "fd_rel_err": errs}
+
+
+def run_mixed_fgmres(a):
+    """A mixed-precision alternative to the automatic FP32→FP64 hand-over,
+    measured on the N³ random-box system: FP64 flexible GMRES(m) whose
+    preconditioner is k steps of the library's FP32-storage SQMR from a zero
+    start (the mixed solve with maxit = k), against the plain FP64 SQMR solve.
+    The outer method runs in FP64 on the device (torch vectors, classical
+    Gram-Schmidt applied twice, the small least-squares problem on the host);
+    it stops at FP64 relative residual 1e-8 or after about 150,000 inner steps."""
+    import numpy as np
+    import torch
+    import maxwellfdfd as M
+    from maxwellfdfd import workloads as W
+    log = lambda *x: print("[mixed-fgmres]", *x, file=sys.stderr, flush=True)
+    N, mres = a.N, a.restart
+    m, om = W.random_boxes(N=N, npml=10, nbox=40, lam=24.0, seed=1, ndip=1)
+    m._bench_srcs = [(pos, v) for pos, v in W.random_dipoles(N, 10, 1, 1001)]
+    M.calc_matparams(m)
+    je, jm = M.create_srcs(m)
+    b = M.create_b(M.EE, om, m, je, jm)
+    op = M.MaxwellOperator(m, om)
+    op.autotune(1, 10)
+    dev = torch.device("cuda:0")
+    bd = torch.from_numpy(b).to(dev)
+    torch.cuda.synchronize()
+    bn = torch.linalg.vector_norm(bd).item()
+
+    def A(x):
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        op.apply_dev(x.data_ptr(), y.data_ptr(), 1, "fp64")
+        return y
+
+    count = {"inner": 0, "fp32": 0}
+
+    def precondition(v, k):
+        z = torch.zeros_like(v)
+        torch.cuda.synchronize()
+        st = op.solve_dev(v.data_ptr(), z.data_ptr(), 1, tol=1e-30, precision="mixed", maxit=k, inner_tol=1e-30,
+                          raise_on_noconv=False)
+        count["inner"] += int(st.iterations)
+        count["fp32"] += int(st.iterations_fp32)
+        return z
+
+    e = torch.zeros_like(bd)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = op.solve_dev(bd.data_ptr(), e.data_ptr(), 1, tol=1e-8)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    log(f"FP64 SQMR {t64:.2f} s, {st.iterations} steps")
+
+    def fgmres(k, max_outer, tol=1e-8):
+        x = torch.zeros_like(bd)
+        outer, hist = 0, []
+        while True:
+            r = bd - A(x)
+            beta = torch.linalg.vector_norm(r).item()
+            hist.append([outer, count["inner"], beta / bn])
+            log(f"k {k} outer {outer} inner steps {count['inner']} FP64 residual {beta / bn:.3e}")
+            if beta / bn <= tol or outer >= max_outer:
+                return x, outer, hist
+            V, Z = [r / beta], []
+            H = np.zeros((mres + 1, mres), complex)
+            y = None
+            for j in range(mres):
+                Z.append(precondition(V[j], k))
+                w = A(Z[j])
+                for _ in range(2):
+                    h = torch.stack([torch.vdot(V[i], w) for i in range(j + 1)]).cpu().numpy()
+                    H[:j + 1, j] += h
+                    for i in range(j + 1):
+                        w = w - complex(h[i]) * V[i]
+                hn = torch.linalg.vector_norm(w).item()
+                H[j + 1, j] = hn
+                outer += 1
+                rhs = np.zeros(j + 2, complex)
+                rhs[0] = beta
+                y = np.linalg.lstsq(H[:j + 2, :j + 1], rhs, rcond=None)[0]
+                est = np.linalg.norm(H[:j + 2, :j + 1] @ y - rhs) / bn
+                if est <= 0.5 * tol or hn == 0 or outer >= max_outer:
+                    break
+                V.append(w / hn)
+            for i in range(len(y)):
+                x = x + complex(y[i]) * Z[i]
+
+    runs = []
+    for k in (int(v) for v in a.inner_steps.split(",")):
+        count["inner"] = count["fp32"] = 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, outer, hist = fgmres(k, max(2, 150000 // k))
+        torch.cuda.synchronize()
+        tf = time.perf_counter() - t0
+        runs.append({"inner_steps_per_application": k, "s": round(tf, 3), "applications": outer,
+                     "inner_steps": count["inner"], "inner_steps_fp32": count["fp32"], "relres": hist[-1][2],
+                     "converged": hist[-1][2] <= 1e-8, "time_vs_fp64_sqmr": round(tf / t64, 3),
+                     "history": hist})
+    op.close()
+    return {"workload": f"mixed-fgmres {N}^3 random boxes: FP64 FGMRES({mres}) preconditioned by k FP32-storage "
+                        f"SQMR steps, against FP64 SQMR, to FP64 rel. residual 1e-8",
+            "fp64_sqmr": {"s": round(t64, 3), "steps": int(st.iterations)}, "fgmres": runs}
 
 
 def run_next_rows(a):
@@ -570,6 +680,8 @@ def main():
             out = run_uneven_batch(a)
         elif a.workload == "large-aniso":
             out = run_large_aniso(a)
+        elif a.workload == "mixed-fgmres":
+            out = run_mixed_fgmres(a)
         elif a.workload == "next-rows":
             out = run_next_rows(a)
         else:

@@ -2081,6 +2081,16 @@ void solve_opts_default(mfd_solve_opts *o) {
     o->use_graph = 1;
 }
 
+// with MFD_SOLVE_TRACE set, a long cycle reports the largest recurrence
+// residual estimate τ/‖c‖ of its running RHS every 512 steps
+static void trace_progress(const Scal *scal_h, int64_t nrhs, long long done_it, long long it) {
+    if (done_it % 512 != 0 || !getenv("MFD_SOLVE_TRACE")) return;
+    double worst = 0;
+    for (int64_t r = 0; r < nrhs; ++r)
+        if (!scal_h[r].conv && scal_h[r].cn > 0) worst = std::max(worst, scal_h[r].tau / scal_h[r].cn);
+    fprintf(stderr, "[mfd step] it %lld recurrence tau/cn %.3e\n", it, worst);
+}
+
 // stalled (optional): set when an FP32 cycle ended because its probes stopped
 // improving, rather than at its target or budget
 template <typename R>
@@ -2144,12 +2154,7 @@ static int run_inner(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double r
         }
         *iters_out = mx;
         if (all || mx >= budget) break;
-        if (done_it % 512 == 0 && getenv("MFD_SOLVE_TRACE")) {   // progress of a long cycle
-            double worst = 0;
-            for (int64_t r = 0; r < nrhs; ++r)
-                if (!w.scal_h[r].conv && w.scal_h[r].cn > 0) worst = std::max(worst, w.scal_h[r].tau / w.scal_h[r].cn);
-            fprintf(stderr, "[mfd step] it %lld recurrence tau/cn %.3e\n", mx, worst);
-        }
+        trace_progress(w.scal_h, nrhs, done_it, mx);
         bool probe_now = false;
         if (sizeof(R) == 4) probe_now = b && e && done_it % probe == 0;
         else if (b && e && tol > 0 && cyc_res > 0 && done_it >= next64) {
@@ -2394,6 +2399,7 @@ static int run_inner_bicgstab(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o,
     MFD_HIP(hipGetLastError());
     int chunk = o->check_every > 0 ? o->check_every : 32;
     bool use_graph = o->use_graph != 0;
+    long long done_it = 0;
     if (use_graph && (w.graph_iters != chunk || w.graph_kind != 4)) {
         if (w.graph) { hipGraphExecDestroy(w.graph); w.graph = nullptr; }
         hipGraph_t gr;
@@ -2420,6 +2426,8 @@ static int run_inner_bicgstab(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o,
         }
         *iters_out = mx;
         if (all || mx >= budget) break;
+        done_it += chunk;
+        trace_progress(w.scal_h, nrhs, done_it, mx);
     }
     return MFD_OK;
 }

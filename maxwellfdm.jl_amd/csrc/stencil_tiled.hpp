@@ -432,7 +432,12 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
         }
         for (int k = k0; k < k1; ++k) {
             const int bk = (k - k0 + 1) & 1, bn = bk ^ 1;
-            __syncthreads();   // buffer bn, Hs and Ps[(k+1)&1] are free
+            // buffer bn, Hs and Ps[(k+1)&1] are free.  Without the grad-div
+            // and off-diagonal terms the output phase reads only Hs, and the
+            // last reads of buffer bn (the previous plane's H) lie before the
+            // barrier that published Hs, so only the prologue's reads of the
+            // first two planes need this one
+            if (GD || OFFD || k == k0) __syncthreads();
             // z factors of plane k (scalar loads: waiting for them does not
             // wait for the prefetch)
             const c64 az_k = ldu(g.sdlinv[DUAL][2], k), ez_k = ldu(g.sdlinv[PRIM][2], k);
