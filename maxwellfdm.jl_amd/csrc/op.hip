@@ -1129,7 +1129,12 @@ static int launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int6
     }
     if (MODE == MODE_P && op->gd && ty == 8) ty = 4;   // the grad-div tile keeps two more LDS planes
     if (!op->has_mu && op->ee_all && ty > 0) {
-        int nx = (g.N[0] + 63) / 64, ny = (g.N[1] + ty - 1) / ty;
+        // order bit 1: 32-wide tiles of 2·TY rows (same threads per block);
+        // the off-diagonal and grad-div stencils are built 64-wide only
+        const bool narrow = (xr & 2) && !op->full && !(MODE == MODE_P && op->gd);
+        xr &= 1;
+        const int tw = narrow ? 32 : 64, rows = narrow ? 2 * ty : ty;
+        int nx = (g.N[0] + tw - 1) / tw, ny = (g.N[1] + rows - 1) / rows;
         int nch = (g.N[2] + kz - 1) / kz;
         while ((int64_t)nx * ny * nch > NPART) { kz *= 2; nch = (g.N[2] + kz - 1) / kz; }
         dim3 grid(nx, ny, (unsigned)(nch * nrhs));
@@ -1142,6 +1147,11 @@ static int launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int6
             if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, true><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
             else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, true><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
             else k_apply_tiled<R, MODE, EPI, 1, true><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+        } else if (narrow) {
+            if (ty == 8) k_apply_tiled<R, MODE, EPI, 16, false, false, false, 32><<<grid, 512, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else if (ty == 2) k_apply_tiled<R, MODE, EPI, 4, false, false, false, 32><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else if (ty == 4) k_apply_tiled<R, MODE, EPI, 8, false, false, false, 32><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else k_apply_tiled<R, MODE, EPI, 2, false, false, false, 32><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
         } else {
             if (ty == 8) k_apply_tiled<R, MODE, EPI, 8, false><<<grid, 512, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
             else if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, false><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
@@ -1648,14 +1658,17 @@ int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best
     // scored by the whole step, so the caches hold what they hold inside a
     // solve (a tile order whose halo rows miss L2 costs the stencil and, through
     // the caches, the vector kernels around it).
+    // 32-wide tiles (orders 2, 3) are candidates where 64-wide tiles would
+    // leave lanes of the last x tile idle
     const int tys[4] = {1, 2, 4, 8}, kzs[4] = {4, 8, 16, 32};
+    const int norders = (op->G.h.N[0] % 64 != 0 && !op->full && !op->gd) ? 4 : 2;
     double bestms[2] = {1e30, 1e30};
     int bt[2][3] = {{4, 16, 1}, {4, 16, 1}};
     const int its = std::max(2, std::min(iters, 50));
     for (int prec = 0; prec < 2; ++prec)
         for (int a = 0; a < 4; ++a)
             for (int b = 0; b < 4; ++b)
-                for (int xo = 0; xo < 2; ++xo) {
+                for (int xo = 0; xo < norders; ++xo) {
                     int *tile = prec == 0 ? op->tile64 : op->tile32;
                     tile[0] = tys[a];
                     tile[1] = kzs[b];
@@ -1854,7 +1867,8 @@ static void launch_cocg_iter(mfd_op *op, int64_t nrhs, int par) {
     int ty = std::is_same<R, double>::value ? op->tile64[0] : op->tile32[0];
     if (!op->csr && !op->full && !op->has_mu && !op->gd && op->ee_all && ty > 0) {
         int kz = std::is_same<R, double>::value ? op->tile64[1] : op->tile32[1];
-        int xr = std::is_same<R, double>::value ? op->tile64[2] : op->tile32[2];
+        // the fused COCG stencil is built 64-wide only: the narrow-tile bit is ignored
+        int xr = (std::is_same<R, double>::value ? op->tile64[2] : op->tile32[2]) & 1;
         int nx = (g.N[0] + 63) / 64, ny = (g.N[1] + ty - 1) / ty, nch = (g.N[2] + kz - 1) / kz;
         while ((int64_t)nx * ny * nch > NPART) { kz *= 2; nch = (g.N[2] + kz - 1) / kz; }
         dim3 grid(nx, ny, (unsigned)(nch * nrhs));
@@ -2059,8 +2073,8 @@ static int run_cocg(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_
 
 int op_set_tile_impl(mfd_op *op, int prec, int ty, int kz, int xcd) {
     if (op->csr) { set_error("tile: an operator built from a CSC matrix has no stencil tiles"); return MFD_ERR_UNSUPPORTED; }
-    if (!(ty == 0 || ty == 1 || ty == 2 || ty == 4 || ty == 8) || kz < 1 || (xcd != 0 && xcd != 1)) {
-        set_error("tile: TY in {0,1,2,4,8}, KZ >= 1, order 0 or 1");
+    if (!(ty == 0 || ty == 1 || ty == 2 || ty == 4 || ty == 8) || kz < 1 || xcd < 0 || xcd > 3) {
+        set_error("tile: TY in {0,1,2,4,8}, KZ >= 1, order in {0,1,2,3} (bit 0: XCD order, bit 1: 32-wide tiles)");
         return MFD_ERR_ARG;
     }
     int *t = prec == MFD_FP64 ? op->tile64 : op->tile32;

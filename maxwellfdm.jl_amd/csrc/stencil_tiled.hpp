@@ -6,8 +6,11 @@
 // against exactly the operator the FP64 residual uses — only the vectors are
 // rounded, once, at the store.  The kernel is HBM-bound: FP64 math is free.
 //
-// A block owns a 64 × TY column tile in x-y (one 64-lane wavefront per x row,
-// so a row's loads are contiguous) and marches over a chunk of KZ z-planes.
+// A block owns a TX × TY column tile in x-y (TX = 64: one 64-lane wavefront
+// per x row, so a row's loads are contiguous; TX = 32: two rows per
+// wavefront, for grids whose x size is not a multiple of 64, where 64-wide
+// tiles would leave lanes of the last tile idle) and marches over a chunk of
+// KZ z-planes.
 // Per plane it stages the scaled iterate X = Rf ∘ x of plane k+1 (tile + a
 // one-cell halo ring) in LDS, computes H = Cₑ X of plane k on the tile plus
 // its −x/−y halo in LDS, and forms y at plane k from H(k), its −x/−y
@@ -22,9 +25,9 @@
 // in LDS (G: forward differences to E edges, Div: backward differences to the
 // primal nodes, the same metric as the curls).  Included by op.hip.
 
-template <int TY>
+template <int TY, int TXW = 64>
 struct TileCfg {
-    static constexpr int TX = 64;
+    static constexpr int TX = TXW;       // tile width: 64, or 32 (two rows per wavefront)
     static constexpr int NT = TX * TY;   // threads per block
     static constexpr int XW = TX + 2;    // staged row width (halo both sides)
     static constexpr int XH = TY + 2;
@@ -126,8 +129,8 @@ __device__ inline void plane_init(const DGrid<R> &g, int vk, PlaneZ<R> &z) {
 // OFFD: the ε tensor has corner off-diagonals (xy, xz, yz); their term is
 // formed from the staged planes k−1 (three X_z values kept in registers),
 // k and k+1, with the corner ε read from global memory.
-template <typename R, int MODE, int EPI, int TY, bool GD, bool FUSE = false, bool OFFD = false>
-__global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData<double> od, c64 om2,
+template <typename R, int MODE, int EPI, int TY, bool GD, bool FUSE = false, bool OFFD = false, int TXW = 64>
+__global__ __launch_bounds__(TXW * TY) void k_apply_tiled(DGrid<double> g, OpData<double> od, c64 om2,
                                                          const cx<R> *__restrict__ xall,
                                                          cx<R> *__restrict__ yall, const c64 *__restrict__ ball,
                                                          int64_t vs, int KZ, int nch, double *part,
@@ -135,7 +138,7 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
                                                          const cx<R> *__restrict__ rall = nullptr,
                                                          cx<R> *__restrict__ pnall = nullptr,
                                                          cx<R> *__restrict__ itall = nullptr) {
-    using C = TileCfg<TY>;
+    using C = TileCfg<TY, TXW>;
     constexpr int GB = GD ? 2 : 1, GH = GD ? C::XH : 1, GW = GD ? C::XW : 1;
     __shared__ c64 Xs[2][3][C::XH][C::XW];
     __shared__ c64 Ys[GB][3][GH][GW];
@@ -175,7 +178,7 @@ __global__ __launch_bounds__(64 * TY) void k_apply_tiled(DGrid<double> g, OpData
         if (scal[rhs].done) return;
         skip = scal[rhs].conv != 0;
     }
-    const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
+    const int tid = threadIdx.x, tx = tid % C::TX, ty = tid / C::TX;
     const int i0 = bx * C::TX, j0 = by * TY;
     const int i = i0 + tx, j = j0 + ty;
     const bool own = i < g.N[0] && j < g.N[1];

@@ -662,7 +662,8 @@ class MaxwellOperator:
 
     def set_tile(self, precision, ty, kz, xcd_order=1):
         """Stencil tile shape: TY rows per block (1, 2, 4, 8; 0 = direct kernel), KZ planes per
-        block, and tile order (1: a contiguous run of tiles per XCD, 0: dispatch order)."""
+        block, and tile order (bit 0: a contiguous run of tiles per XCD, else dispatch order;
+        bit 1: 32-wide tiles of 2·TY rows, for diagonal ε)."""
         check(lib().mfd_op_set_tile(self._h, L.MFD_FP64 if precision == "fp64" else L.MFD_MIXED, ty, kz,
                                     xcd_order))
 

@@ -224,12 +224,15 @@ OPCASES = ["uniform_pml", "nonuniform_bloch", "mixed_pml_bloch", "tiny_bloch_axe
 
 @pytest.mark.parametrize("case", ["wide", "wide_bloch", "nonuniform_bloch", "tiny_bloch_axes", "mixed_pml_bloch"])
 @pytest.mark.parametrize("ty,kz,xo", [(1, 1, 1), (1, 4, 0), (2, 3, 1), (2, 16, 0), (4, 2, 1), (4, 16, 0),
-                                      (4, 16, 1), (8, 3, 1), (8, 16, 0), (0, 1, 0)])
+                                      (4, 16, 1), (8, 3, 1), (8, 16, 0), (0, 1, 0),
+                                      (1, 3, 3), (2, 5, 2), (4, 16, 3), (8, 2, 3)])
 @pytest.mark.parametrize("full", [False, True])
 def test_apply_tile_shapes(case, ty, kz, xo, full, pkg, O):
     """Every tile shape and tile order of the stencil (and the direct kernel,
     TY = 0) gives the CSC product, with diagonal and with full ε (the corner
-    off-diagonal terms read planes k−1, k, k+1 and wrap with the Bloch phase)."""
+    off-diagonal terms read planes k−1, k, k+1 and wrap with the Bloch phase).
+    Orders 2 and 3 ask for 32-wide tiles, which the diagonal-ε stencil uses
+    and the full-ε one ignores."""
     m = make(pkg, case)
     eps, muu = rand_params(m, 8, full=full)
     om = 0.55
@@ -461,6 +464,31 @@ def test_solve_vs_direct(precision, pkg, O):
     xd = sla.spsolve(A.tocsc(), b)
     assert np.linalg.norm(e - xd) / np.linalg.norm(xd) <= 1e-6
     op.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_narrow_tiles(precision, pkg, O):
+    """Solves with 32-wide stencil tiles (x size 36: one full tile and one of
+    4 columns) against the oracle's A and the same solve with 64-wide tiles;
+    two right-hand sides, so the RHS-interleaved tile order runs too."""
+    m, om, b = c1_like(pkg, N=36, npml=4, r=7.0, lam=16.0)
+    A = O.to_scipy(O.create_A(spec_of(m), om, m.εarr, m.μarr))
+    B = np.array([b, 1j * b[::-1].copy()])
+    out = {}
+    for xo in (1, 3):
+        op = pkg.MaxwellOperator(m, om)
+        op.set_tile("fp64", 4, 8, xo)
+        op.set_tile("mixed", 2, 4, xo)
+        say("narrow tiles", precision, xo)
+        E, st = op.solve(B, tol=1e-9, precision=precision)
+        op.close()
+        for r in range(2):
+            rel = np.linalg.norm(B[r] - A @ E[r]) / np.linalg.norm(B[r])
+            say("  order", xo, "rhs", r, "iterations", st.iterations, "oracle residual", rel)
+            assert st.converged and rel <= 1.05e-9
+        out[xo] = E
+    for r in range(2):
+        assert np.linalg.norm(out[3][r] - out[1][r]) <= 1e-7 * np.linalg.norm(out[1][r])
 
 
 @pytest.mark.parametrize("kw", [dict(), dict(use_graph=False, check_every=5), dict(check_every=7),
