@@ -221,7 +221,7 @@ int mfd_op_set_graddiv(mfd_op *op, int enable);
 /* Tile-shape autotuner for the stencil (rows per block TY, z-planes per block
  * KZ, tile order: bit 0 set = each XCD gets a contiguous run of tiles, clear =
  * dispatch order; bit 1 set = 32-wide tiles of 2·TY rows, a candidate when
- * the grid's x size is not a multiple of 64 and ε is diagonal): times each
+ * ε is diagonal): times each
  * candidate inside complete solver iterations for nrhs
  * right-hand sides (iters of them, on the solver workspace) and keeps the one
  * with the fastest iteration per precision; ms[] is that iteration time.
@@ -231,8 +231,10 @@ int mfd_op_autotune(mfd_op *op, int64_t nrhs, int iters, int tile64[3], int tile
  * kernel), KZ >= 1, xcd_order in {0,1,2,3} (bit 0: XCD tile order; bit 1:
  * 32-wide tiles of 2·TY rows, used by the diagonal-ε stencils and ignored by
  * the off-diagonal, grad-div and fused COCG ones).  Until this or mfd_op_autotune is
- * called, an operator uses TY = 4, KZ = 16, xcd_order 1, with KZ and then TY
- * halved (down to 4 and 1) while a launch would have fewer than 256 tiles. */
+ * called, an operator uses TY = 4, KZ = 16, XCD order, with KZ and then TY
+ * halved (down to 4 and 1) while a launch would have fewer than 256 tiles,
+ * and 32-wide tiles (xcd_order 3) when ε is diagonal and they leave fewer
+ * lanes of the last x tile idle than 64-wide ones. */
 int mfd_op_set_tile(mfd_op *op, int prec, int ty, int kz, int xcd_order);
 /* Average device time (ms) of one apply of the solver's stencil kernel
  * (MFD_FP64: A; MFD_MIXED: the complex-FP32 inner operator) */

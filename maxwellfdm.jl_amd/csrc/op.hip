@@ -1337,16 +1337,21 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
     else {
         // default tile (until mfd_op_autotune or mfd_op_set_tile): 4 rows x 16
         // planes, with fewer planes and then fewer rows per block on grids too
-        // small for that to give one tile per CU (256)
-        const int64_t nx = (op->G.h.N[0] + 63) / 64;
+        // small for that to give one tile per CU (256); 32-wide tiles (of
+        // twice the rows) where they leave fewer x lanes idle than 64-wide ones
+        const int64_t N0 = op->G.h.N[0];
+        const bool narrow = !full && 64 * ((N0 + 63) / 64) - N0 > 32 * ((N0 + 31) / 32) - N0;
+        const int64_t nx = narrow ? (N0 + 31) / 32 : (N0 + 63) / 64;
         auto tiles = [&](int ty, int kz) {
-            return nx * ((op->G.h.N[1] + ty - 1) / ty) * ((op->G.h.N[2] + kz - 1) / kz);
+            const int rows = narrow ? 2 * ty : ty;
+            return nx * ((op->G.h.N[1] + rows - 1) / rows) * ((op->G.h.N[2] + kz - 1) / kz);
         };
         int ty = 4, kz = 16;
         while (kz > 4 && tiles(ty, kz) < 256) kz /= 2;
         while (ty > 1 && tiles(ty, kz) < 256) ty /= 2;
         op->tile64[0] = op->tile32[0] = ty;
         op->tile64[1] = op->tile32[1] = kz;
+        op->tile64[2] = op->tile32[2] = narrow ? 3 : 1;
     }
     op->gd = 0;   // opt-in: it helps small PML-dominated grids and slows larger ones
     k_wnode<<<nb(M), BS, 0, s>>>(op->G.d64, epsd, wn, wn32);
@@ -1658,10 +1663,10 @@ int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best
     // scored by the whole step, so the caches hold what they hold inside a
     // solve (a tile order whose halo rows miss L2 costs the stencil and, through
     // the caches, the vector kernels around it).
-    // 32-wide tiles (orders 2, 3) are candidates where 64-wide tiles would
-    // leave lanes of the last x tile idle
+    // 32-wide tiles (orders 2, 3) are candidates for the diagonal-ε stencil
+    // (they pay where 64-wide tiles would leave lanes of the last x tile idle)
     const int tys[4] = {1, 2, 4, 8}, kzs[4] = {4, 8, 16, 32};
-    const int norders = (op->G.h.N[0] % 64 != 0 && !op->full && !op->gd) ? 4 : 2;
+    const int norders = (!op->full && !op->gd) ? 4 : 2;
     double bestms[2] = {1e30, 1e30};
     int bt[2][3] = {{4, 16, 1}, {4, 16, 1}};
     const int its = std::max(2, std::min(iters, 50));
