@@ -181,6 +181,31 @@ def kernel_table(op, cells, iters=50):
     return out
 
 
+def copy_rates(sizes_mib=(64, 256, 1024, 4096), iters=20):
+    """Device-to-device copy rate (GB/s, bytes read + written) at several sizes:
+    the practical streaming ceiling next to the 8 TB/s peak.  Working sets up
+    to the 256 MiB Infinity Cache run above the HBM rate."""
+    import torch
+    dev = torch.device("cuda:0")
+    out = {}
+    for mb in sizes_mib:
+        src = torch.ones(mb * 262144, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(iters):
+            dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / iters
+        out[f"{mb}MiB"] = round(2 * mb * 1048576 / (ms * 1e-3) / 1e9, 1)
+        del src, dst
+    torch.cuda.empty_cache()
+    return out
+
+
 def run_batched_rhs(a):
     """N³ random boxes (the 40-box generator) with nrhs seeded dipoles solved in
     one batched launch (FP64 or mixed), against one single-RHS solve of the
@@ -757,6 +782,7 @@ def main():
 
     # curl-curl apply bandwidth (FP64 A, 144 B/cell) and the iteration's kernels
     ms_apply = op.bench_apply(1, "fp64", iters=50)
+    copies = copy_rates()
     gbps_apply = 144.0 * cells / (ms_apply * 1e-3) / 1e9
     kt = kernel_table(op, cells)
     # dominant kernel: the largest share of the iteration in the precision
@@ -801,6 +827,7 @@ def main():
         "speedup_mixed_vs_fp64": round(t_fp64 / t_mixed, 3),
         "apply": {"fp64_A_ms": round(ms_apply, 5), "fp64_A_GBps": round(gbps_apply, 1),
                   "tiles_fp64_fp32": [list(tiles[0]), list(tiles[1])]},
+        "device_copy_GBps": copies,
         "kernels": kt,
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom].format(R=rdev),
                      "achieved": kd["GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
