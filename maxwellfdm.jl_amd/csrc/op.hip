@@ -410,7 +410,7 @@ __global__ __launch_bounds__(BS) void k_apply(DGrid<R> g, OpData<R> od, const cx
                 if (EPI == EPI_RESID) {
                     c64 bb = ball[rhs * vs + e];
                     cx<R> rr = mk<R>((R)bb.re - out.re, (R)bb.im - out.im);
-                    y[e] = rr;
+                    if (yall) y[e] = rr;   // null output: the residual norm only
                     acc[0] += (double)rr.re * rr.re + (double)rr.im * rr.im;
                 } else {
                     y[e] = out;
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int64_t *__restric
             if (EPI == EPI_RESID) {
                 const c64 bb = ball[rhs * vs + i];
                 const c64 rr = mk<double>(bb.re - out.re, bb.im - out.im);
-                y[i] = cvt<R>(rr);
+                if (yall) y[i] = cvt<R>(rr);   // null output: the residual norm only
                 acc[0] += rr.re * rr.re + rr.im * rr.im;
             } else {
                 y[i] = cvt<R>(out);
@@ -2404,9 +2404,20 @@ static void launch_bicgstab_iter(mfd_op *op, int64_t nrhs) {
     k_bi_p<<<gv, BS, 0, s>>>(pp, r, v, n, n, w.scal);
 }
 
-// one BiCGSTAB cycle from the FP64 residual in w.r64; x accumulates in w.z
+// one BiCGSTAB cycle from the FP64 residual in w.r64; x accumulates in w.z.
+// Given b, e and the host per-RHS state (‖b‖², best, active), the cycle
+// probes the true FP64 residual of e + x every `probe` steps (w.r64, free
+// once the cycle has started, holds the probed iterate; the residual vector
+// itself is not stored): a column whose probe beats its best residual so far
+// is copied to w.ebest and its best lowered, the cycle ends when every
+// running column meets 0.9·tol on the probe, and it ends early — for a restart
+// from the best iterate — when the probed residual has risen 10× above the
+// cycle's best probe (BiCGSTAB's erratic phases, in which its updated
+// residual also drifts from the true one).
 static int run_inner_bicgstab(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o, double rel_target,
-                              const int *active_d, long long budget, long long *iters_out) {
+                              const int *active_d, long long budget, long long *iters_out,
+                              const void *b = nullptr, const void *e = nullptr, const double *bn = nullptr,
+                              double *best = nullptr, const int *active_h = nullptr) {
     Workspace &w = op->ws;
     hipStream_t s = op->stream;
     int64_t n = w.n;
@@ -2419,6 +2430,8 @@ static int run_inner_bicgstab(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o,
     int chunk = o->check_every > 0 ? o->check_every : 32;
     bool use_graph = o->use_graph != 0;
     long long done_it = 0;
+    const long long probe = std::max<long long>(chunk, 2048) / chunk * chunk;
+    double cyc_best = HUGE_VAL;   // smallest worst-column probe of this cycle
     if (use_graph && (w.graph_iters != chunk || w.graph_kind != 4)) {
         if (w.graph) { hipGraphExecDestroy(w.graph); w.graph = nullptr; }
         hipGraph_t gr;
@@ -2447,6 +2460,30 @@ static int run_inner_bicgstab(mfd_op *op, int64_t nrhs, const mfd_solve_opts *o,
         if (all || mx >= budget) break;
         done_it += chunk;
         trace_progress(w.scal_h, nrhs, done_it, mx);
+        if (!(b && e && bn && best && active_h) || done_it % probe) continue;
+        k_probe_x<double><<<gv, BS, 0, s>>>(op->G.d64, op->d64, (const c64 *)e, (const c64 *)w.z, (c64 *)w.r64, n, n);
+        launch_apply<double, MODE_A, EPI_RESID>(op, (const c64 *)w.r64, nullptr, (const c64 *)b, nrhs,
+                                                w.part + 2 * pslot, nullptr, w.rn, active_d);
+        MFD_HIP(hipGetLastError());
+        MFD_HIP(hipMemcpyAsync(w.rn_h, w.rn, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+        MFD_HIP(hipStreamSynchronize(s));
+        double worst = 0;
+        bool done = true;
+        for (int64_t r = 0; r < nrhs; ++r) {
+            if (!active_h[r] || w.scal_h[r].conv) continue;
+            const double rel = rel_residual(w.rn_h[r], bn[r]);
+            if (rel < best[r]) {
+                best[r] = rel;
+                MFD_HIP(hipMemcpyAsync((c64 *)w.ebest + r * n, (const c64 *)w.r64 + r * n, sizeof(c64) * n,
+                                       hipMemcpyDeviceToDevice, s));
+            }
+            worst = std::isnan(rel) || std::isnan(worst) ? NAN : std::max(worst, rel);
+            if (!(rel <= 0.9 * o->tol)) done = false;
+        }
+        if (getenv("MFD_SOLVE_TRACE")) fprintf(stderr, "[mfd probe] it %lld true residual %.3e\n", mx, worst);
+        if (done) break;
+        if (std::isnan(worst) || worst > 10.0 * cyc_best) break;
+        cyc_best = std::min(cyc_best, worst);
     }
     return MFD_OK;
 }
@@ -2527,7 +2564,7 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
     // improving (the FP32 recurrence stagnated; a restart from the refined
     // residual stagnates again), switches the rest of the solve to FP64 cycles
     int cyc_prec = method == MFD_BICGSTAB ? MFD_FP64 : prec;
-    bool stalled32 = false;
+    bool stalled32 = false, restored = false;
     while (true) {
         launch_apply<double, MODE_A, EPI_RESID>(op, (const c64 *)e, (c64 *)w.r64, (const c64 *)b, nrhs,
                                                 w.part + 2 * pslot, nullptr, w.rn);
@@ -2555,6 +2592,19 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
         worst = max_rel(rel);
         if (all) { conv = true; break; }
         if (!any_active || total_it >= o.maxit || cycles > 200) break;
+        // BiCGSTAB: a column whose cycle ended above twice its best iterate (a
+        // probe snapshot or an earlier cycle start) restarts from that iterate;
+        // its residual is then recomputed, and equals the best exactly
+        if (method == MFD_BICGSTAB && !restored) {
+            for (int64_t r = 0; r < nrhs; ++r) {
+                if (!active[r] || best[r] == HUGE_VAL || !(rel[r] > 2.0 * best[r])) continue;
+                MFD_HIP(hipMemcpyAsync((c64 *)e + r * n, (const c64 *)w.ebest + r * n, sizeof(c64) * n,
+                                       hipMemcpyDeviceToDevice, s));
+                restored = true;
+            }
+            if (restored) continue;
+        }
+        restored = false;
         if (cyc_prec == MFD_MIXED && cycles > 0 && (worst > 0.1 * prev_worst || stalled32)) cyc_prec = MFD_FP64;
         prev_worst = worst;
         MFD_HIP(hipMemcpyAsync(active_d, active.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s));
@@ -2572,7 +2622,9 @@ int solve_dev_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_s
         if (cyc_prec == MFD_MIXED) target = std::max(target, o.inner_tol);
         long long it = 0;
         int rc;
-        if (method == MFD_BICGSTAB) rc = run_inner_bicgstab(op, nrhs, &o, target, active_d, o.maxit - total_it, &it);
+        if (method == MFD_BICGSTAB)
+            rc = run_inner_bicgstab(op, nrhs, &o, target, active_d, o.maxit - total_it, &it, b, e, bn.data(),
+                                    best.data(), active.data());
         else if (cyc_prec == MFD_MIXED)
             rc = run_inner<float>(op, nrhs, &o, target, active_d, o.maxit - total_it, &it, b, e, bn.data(), &stalled32);
         else
