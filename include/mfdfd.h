@@ -254,7 +254,12 @@ int mfd_op_bench_iteration(mfd_op *op, int64_t nrhs, int prec, int iters, double
  * stats.relres NaN.  A SQMR or BiCGSTAB solve that ends without converging
  * (MFD_ERR_NOCONV) leaves in each column of e the iterate with that column's
  * smallest such residual (converged columns keep their converged iterate);
- * stats.relres reports the largest of those residuals. */
+ * stats.relres reports the largest of those residuals.  BiCGSTAB cycles also
+ * probe that residual every 2,048 steps: a probe that improves a column's
+ * best keeps its iterate as the best, the cycle ends once every column meets
+ * 0.9·tol on a probe or the probed residual rises 10× above the cycle's best
+ * probe, and a column that ends a cycle above twice its best starts the next
+ * cycle from its best iterate. */
 void mfd_solve_opts_default(mfd_solve_opts *o);
 int mfd_solve(mfd_op *op, const mfd_c64 *b, mfd_c64 *e, int64_t nrhs, const mfd_solve_opts *o,
               mfd_solve_stats *st);
