@@ -350,7 +350,9 @@ __global__ __launch_bounds__(TXW * TY) void k_apply_tiled(DGrid<double> g, OpDat
 
     // H of an interior plane (buffers bk: that plane, bn: the next) for all H
     // points; az is 1/s∆l_z of the plane
-    auto hplane = [&](int bk, int bn, c64 az) {
+    // slab: a one-cell z axis, whose plane k+1 is plane k times pzn (the
+    // Bloch phase, or 0 past a non-Bloch end) and is not staged
+    auto hplane = [&](int bk, int bn, c64 az, bool slab, c64 pzn) {
 #pragma unroll
         for (int r = 0; r < NHP; ++r) {
             int t = tid + r * C::NT;
@@ -360,8 +362,10 @@ __global__ __launch_bounds__(TXW * TY) void k_apply_tiled(DGrid<double> g, OpDat
             if (hok[r]) {
                 c64 ax = hax[r], ay = hay[r];
                 c64 X0x = Xs[bk][0][hj][hi], X0y = Xs[bk][1][hj][hi], X0z = Xs[bk][2][hj][hi];
-                h0 = ay * (Xs[bk][2][hj + 1][hi] - X0z) - az * (Xs[bn][1][hj][hi] - X0y);
-                h1 = az * (Xs[bn][0][hj][hi] - X0x) - ax * (Xs[bk][2][hj][hi + 1] - X0z);
+                const c64 Xnx = slab ? pzn * X0x : Xs[bn][0][hj][hi];
+                const c64 Xny = slab ? pzn * X0y : Xs[bn][1][hj][hi];
+                h0 = ay * (Xs[bk][2][hj + 1][hi] - X0z) - az * (Xny - X0y);
+                h1 = az * (Xnx - X0x) - ax * (Xs[bk][2][hj][hi + 1] - X0z);
                 h2 = ax * (Xs[bk][1][hj][hi + 1] - X0y) - ay * (Xs[bk][0][hj + 1][hi] - X0x);
             }
             Hs[0][hj][hi] = h0;
@@ -400,6 +404,10 @@ __global__ __launch_bounds__(TXW * TY) void k_apply_tiled(DGrid<double> g, OpDat
     c64 hpx = czero<double>(), hpy = czero<double>();
     c64 zkm0 = czero<double>(), zkmx = czero<double>(), zkmy = czero<double>();
     if (!skip && k0 < k1) {
+        // a one-cell z axis (an extruded 2D model): its single plane is staged
+        // once; its neighbours are that plane times the Bloch phase, or zero
+        const bool slab = PIPE && g.N[2] == 1;
+        const c64 pzn = g.bloch[2] ? g.ph[2] : czero<double>();
         // the owner's factors of H_x, H_y at plane k0−1 (loaded ahead of the
         // prefetch, as every per-thread load in the pipelined loop is)
         const bool hpv = own && (k0 > 0 || g.bloch[2]);
@@ -409,7 +417,10 @@ __global__ __launch_bounds__(TXW * TY) void k_apply_tiled(DGrid<double> g, OpDat
             pax = g.sdlinv[DUAL][0][i];
             pay = g.sdlinv[DUAL][1][j];
         }
-        if (PIPE) {
+        if (slab) {   // one plane, staged once into buffer 1
+            fetch(0);
+            commit(1, 0, xc, qc, ec);
+        } else if (PIPE) {
             fetch(k0 - 1);
             commit(0, k0 - 1, xn, qn, en);
             fetch(k0);
@@ -421,7 +432,13 @@ __global__ __launch_bounds__(TXW * TY) void k_apply_tiled(DGrid<double> g, OpDat
         }
         __syncthreads();
         // H_x, H_y of plane k0−1 at the owned point (zero across a non-Bloch z ghost)
-        if (hpv) {
+        if (hpv && slab) {   // plane −1 is plane 0 times e^{+ikL}, the inverse phase
+            int hi = tx + 1, hj = ty + 1;
+            const c64 pm = g.phinv[2], dm = cone<double>() - pm;
+            c64 X0x = Xs[1][0][hj][hi], X0y = Xs[1][1][hj][hi], X0z = Xs[1][2][hj][hi];
+            hpx = pay * (pm * (Xs[1][2][hj + 1][hi] - X0z)) - paz * (dm * X0y);
+            hpy = paz * (dm * X0x) - pax * (pm * (Xs[1][2][hj][hi + 1] - X0z));
+        } else if (hpv) {
             int hi = tx + 1, hj = ty + 1;
             c64 X0x = Xs[0][0][hj][hi], X0y = Xs[0][1][hj][hi], X0z = Xs[0][2][hj][hi];
             hpx = pay * (Xs[0][2][hj + 1][hi] - X0z) - paz * (Xs[1][1][hj][hi] - X0y);
@@ -446,13 +463,14 @@ __global__ __launch_bounds__(TXW * TY) void k_apply_tiled(DGrid<double> g, OpDat
             const c64 az_k = ldu(g.sdlinv[DUAL][2], k), ez_k = ldu(g.sdlinv[PRIM][2], k);
             const c64 tzm = axis_term_u(g, k);
             const c64 sdD = ldu(g.sdl[DUAL][2], k), sdP = ldu(g.sdl[PRIM][2], k);
-            if (PIPE) {
+            if (slab) {
+            } else if (PIPE) {
                 commit(bn, k + 1, xn, qn, en);
                 // past the chunk: an out-of-ring plane, which loads nothing
                 fetch(k + 2 <= k1 ? k + 2 : -2);
             } else stage(bn, k + 1, xn, qn, en);
             __syncthreads();
-            hplane(bk, bn, az_k);
+            hplane(bk, bn, az_k, slab, pzn);
             if (GD) pplane(bn, bk, k + 1, (k + 1) & 1);
             __syncthreads();
             if (own) {

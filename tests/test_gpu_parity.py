@@ -40,6 +40,11 @@ CASES = {
     "wide": dict(N=(70, 5, 7), npml=((3, 1, 1), (2, 1, 2)), om=0.45, bloch=(False,) * 3),
     "wide_bloch": dict(N=(67, 3, 5), npml=((0, 0, 1), (0, 0, 1)), om=0.5, bloch=(True, True, False),
                        kb=(0.02, 0.3, 0.0)),
+    # one-cell z axes (the extruded 2D models), with a z phase and without
+    "slab_bloch": dict(N=(9, 7, 1), npml=((2, 1, 0), (1, 2, 0)), om=0.6, bloch=(False, False, True),
+                       kb=(0.0, 0.0, 0.35)),
+    "slab_2d": dict(N=(66, 5, 1), npml=((0, 1, 0), (0, 1, 0)), om=0.5, bloch=(True, False, True),
+                    kb=(0.1, 0.0, 0.0)),
 }
 
 
@@ -222,7 +227,8 @@ def test_matparams(nonuni, pkg, O):
 OPCASES = ["uniform_pml", "nonuniform_bloch", "mixed_pml_bloch", "tiny_bloch_axes", "cmplast", "wide"]
 
 
-@pytest.mark.parametrize("case", ["wide", "wide_bloch", "nonuniform_bloch", "tiny_bloch_axes", "mixed_pml_bloch"])
+@pytest.mark.parametrize("case", ["wide", "wide_bloch", "nonuniform_bloch", "tiny_bloch_axes", "mixed_pml_bloch",
+                                  "slab_bloch", "slab_2d"])
 @pytest.mark.parametrize("ty,kz,xo", [(1, 1, 1), (1, 4, 0), (2, 3, 1), (2, 16, 0), (4, 2, 1), (4, 16, 0),
                                       (4, 16, 1), (8, 3, 1), (8, 16, 0), (0, 1, 0),
                                       (1, 3, 3), (2, 5, 2), (4, 16, 3), (8, 2, 3)])
