@@ -144,6 +144,10 @@ __global__ __launch_bounds__(TXW * TY) void k_apply_tiled(DGrid<double> g, OpDat
     __shared__ c64 Ys[GB][3][GH][GW];
     __shared__ c64 Hs[3][C::HHt][C::HW];
     __shared__ c64 Ps[GB][GD ? C::PHt : 1][GD ? C::PW : 1];
+    // OFFD: the corner-average weights of the tile's columns (x) and rows (y),
+    // w0 = ∆l_dual(i)/(2 ∆l_prim(i)) and w1 = ∆l_dual(i−1)/(2 ∆l_prim(i))
+    __shared__ c64 Wx[2][OFFD ? C::TX : 1];
+    __shared__ c64 Wy[2][OFFD ? TY : 1];
     // Tile of this block.  Workgroups are dispatched to the 8 XCDs round-robin
     // (dispatch index L goes to XCD L mod 8); with xcd != 0 each XCD gets one
     // contiguous run of tiles in x-then-y order, so the y halo rows a tile
@@ -219,6 +223,21 @@ __global__ __launch_bounds__(TXW * TY) void k_apply_tiled(DGrid<double> g, OpDat
         T2xy[c] = g.sdl[c == 0 ? DUAL : PRIM][0][io] * g.sdl[c == 1 ? DUAL : PRIM][1][jo];
     const c64 tmx = axis_term(g, 0, io), tmy = axis_term(g, 1, jo);
     const c64 gdx = g.sdlinv[DUAL][0][io], gdy = g.sdlinv[DUAL][1][jo];
+    if (OFFD) {   // published by the prologue's barrier
+        const c64 h5 = mk<double>(0.5, 0);
+        if (tid < C::TX) {
+            const int ii = min(i0 + tid, g.N[0] - 1), im = ii > 0 ? ii - 1 : g.N[0] - 1;
+            const c64 ix = g.sdlinv[PRIM][0][ii];
+            Wx[0][tid] = (h5 * g.sdl[DUAL][0][ii]) * ix;
+            Wx[1][tid] = (h5 * g.sdl[DUAL][0][im]) * ix;
+        }
+        if (tid < TY) {
+            const int jj = min(j0 + tid, g.N[1] - 1), jm = jj > 0 ? jj - 1 : g.N[1] - 1;
+            const c64 iy = g.sdlinv[PRIM][1][jj];
+            Wy[0][tid] = (h5 * g.sdl[DUAL][1][jj]) * iy;
+            Wy[1][tid] = (h5 * g.sdl[DUAL][1][jm]) * iy;
+        }
+    }
 
     // H points of this thread (fixed across planes): LDS (hi, hj) -> global (i0-1+hi, j0-1+hj)
     constexpr int NHP = (C::HHt * C::HW + C::NT - 1) / C::NT;
@@ -498,15 +517,12 @@ __global__ __launch_bounds__(TXW * TY) void k_apply_tiled(DGrid<double> g, OpDat
                 int64_t p = i + (int64_t)g.N[0] * j + NxNy * k;
                 c64 term[3] = {czero<double>(), czero<double>(), czero<double>()};
                 if (OFFD) {
-                    // ∆l weights of the corner averages along each axis (reloaded
-                    // per plane from the cached 1D tables: fewer live registers)
+                    // ∆l weights of the corner averages along each axis: x and y
+                    // from the block's LDS tables, z from the plane's scalar loads
                     const c64 h5 = mk<double>(0.5, 0);
-                    const int im = i > 0 ? i - 1 : g.N[0] - 1, jm = j > 0 ? j - 1 : g.N[1] - 1;
                     const int km = k > 0 ? k - 1 : g.N[2] - 1;
-                    const c64 ix = g.sdlinv[PRIM][0][i], iy = g.sdlinv[PRIM][1][j], iz = g.sdlinv[PRIM][2][k];
-                    const c64 wx0 = (h5 * g.sdl[DUAL][0][i]) * ix, wx1 = (h5 * g.sdl[DUAL][0][im]) * ix;
-                    const c64 wy0 = (h5 * g.sdl[DUAL][1][j]) * iy, wy1 = (h5 * g.sdl[DUAL][1][jm]) * iy;
-                    const c64 wz0 = (h5 * g.sdl[DUAL][2][k]) * iz, wz1 = (h5 * g.sdl[DUAL][2][km]) * iz;
+                    const c64 wx0 = Wx[0][tx], wx1 = Wx[1][tx], wy0 = Wy[0][ty], wy1 = Wy[1][ty];
+                    const c64 wz0 = (h5 * sdD) * ez_k, wz1 = (h5 * ldu(g.sdl[DUAL][2], km)) * ez_k;
                     const int hi = tx + 1, hj = ty + 1;
                     // corner cells p + û (wrapped), or -1 past a non-Bloch end
                     int64_t cpx = i + 1 < g.N[0] ? p + 1 : (g.bloch[0] ? p - (g.N[0] - 1) : -1);
