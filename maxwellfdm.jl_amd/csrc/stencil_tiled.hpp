@@ -148,6 +148,14 @@ __global__ __launch_bounds__(TXW * TY) void k_apply_tiled(DGrid<double> g, OpDat
     // w0 = ∆l_dual(i)/(2 ∆l_prim(i)) and w1 = ∆l_dual(i−1)/(2 ∆l_prim(i))
     __shared__ c64 Wx[2][OFFD ? C::TX : 1];
     __shared__ c64 Wy[2][OFFD ? TY : 1];
+    // OFFD: the owner's per-column and per-row 1D factors (1/∆l_p, 1/∆l_d, the
+    // axis term, ∆l_d, ∆l_p) and the H points' 1/∆l_d, read from block tables
+    // every plane instead of held in registers (the full-ε stencil is the one
+    // short of registers)
+    __shared__ c64 Fx[OFFD ? 5 : 1][OFFD ? C::TX : 1];
+    __shared__ c64 Fy[OFFD ? 5 : 1][OFFD ? TY : 1];
+    __shared__ c64 Hx[OFFD ? C::HW : 1];
+    __shared__ c64 Hy[OFFD ? C::HHt : 1];
     // Tile of this block.  Workgroups are dispatched to the 8 XCDs round-robin
     // (dispatch index L goes to XCD L mod 8); with xcd != 0 each XCD gets one
     // contiguous run of tiles in x-then-y order, so the y halo rows a tile
@@ -215,15 +223,46 @@ __global__ __launch_bounds__(TXW * TY) void k_apply_tiled(DGrid<double> g, OpDat
     }
     // owner's output factors (x from the column, y from the row)
     const int io = own ? i : 0, jo = own ? j : 0;
-    const c64 exo = g.sdlinv[PRIM][0][io], eyo = g.sdlinv[PRIM][1][jo];
+    c64 exo = cone<double>(), eyo = cone<double>(), tmx = czero<double>(), tmy = czero<double>();
+    c64 gdx = czero<double>(), gdy = czero<double>();
     // x-y part of T² (T = S_E^{1/2}: the product of the stretched ∆l)
-    c64 T2xy[3];
+    c64 T2xy[3] = {cone<double>(), cone<double>(), cone<double>()};
+    if (!OFFD) {
+        exo = g.sdlinv[PRIM][0][io];
+        eyo = g.sdlinv[PRIM][1][jo];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-        T2xy[c] = g.sdl[c == 0 ? DUAL : PRIM][0][io] * g.sdl[c == 1 ? DUAL : PRIM][1][jo];
-    const c64 tmx = axis_term(g, 0, io), tmy = axis_term(g, 1, jo);
-    const c64 gdx = g.sdlinv[DUAL][0][io], gdy = g.sdlinv[DUAL][1][jo];
+        for (int c = 0; c < 3; ++c)
+            T2xy[c] = g.sdl[c == 0 ? DUAL : PRIM][0][io] * g.sdl[c == 1 ? DUAL : PRIM][1][jo];
+        tmx = axis_term(g, 0, io);
+        tmy = axis_term(g, 1, jo);
+        gdx = g.sdlinv[DUAL][0][io];
+        gdy = g.sdlinv[DUAL][1][jo];
+    }
     if (OFFD) {   // published by the prologue's barrier
+        if (tid < C::TX) {
+            const int ii = min(i0 + tid, g.N[0] - 1);
+            Fx[0][tid] = g.sdlinv[PRIM][0][ii];
+            Fx[1][tid] = g.sdlinv[DUAL][0][ii];
+            Fx[2][tid] = axis_term(g, 0, ii);
+            Fx[3][tid] = g.sdl[DUAL][0][ii];
+            Fx[4][tid] = g.sdl[PRIM][0][ii];
+        }
+        if (tid < TY) {
+            const int jj = min(j0 + tid, g.N[1] - 1);
+            Fy[0][tid] = g.sdlinv[PRIM][1][jj];
+            Fy[1][tid] = g.sdlinv[DUAL][1][jj];
+            Fy[2][tid] = axis_term(g, 1, jj);
+            Fy[3][tid] = g.sdl[DUAL][1][jj];
+            Fy[4][tid] = g.sdl[PRIM][1][jj];
+        }
+        for (int t = tid; t < C::HW; t += C::NT) {   // HW = TX + 1 exceeds the 64 threads of TY = 1
+            const int ri = i0 - 1 + t;
+            Hx[t] = g.sdlinv[DUAL][0][ri < 0 ? g.N[0] - 1 : (ri < g.N[0] ? ri : 0)];
+        }
+        if (tid < C::HHt) {
+            const int rj = j0 - 1 + tid;
+            Hy[tid] = g.sdlinv[DUAL][1][rj < 0 ? g.N[1] - 1 : (rj < g.N[1] ? rj : 0)];
+        }
         const c64 h5 = mk<double>(0.5, 0);
         if (tid < C::TX) {
             const int ii = min(i0 + tid, g.N[0] - 1), im = ii > 0 ? ii - 1 : g.N[0] - 1;
@@ -245,6 +284,9 @@ __global__ __launch_bounds__(TXW * TY) void k_apply_tiled(DGrid<double> g, OpDat
     bool hok[NHP];
 #pragma unroll
     for (int r = 0; r < NHP; ++r) {
+        hok[r] = false;
+        hax[r] = hay[r] = czero<double>();
+        if (OFFD) continue;   // from the tables and the position, in hplane
         int t = tid + r * C::NT;
         int hj = t / C::HW, hi = t - hj * C::HW;
         int ri = i0 - 1 + hi, rj = j0 - 1 + hj;
@@ -378,8 +420,15 @@ __global__ __launch_bounds__(TXW * TY) void k_apply_tiled(DGrid<double> g, OpDat
             if (t >= C::HHt * C::HW) break;
             int hj = t / C::HW, hi = t - hj * C::HW;
             c64 h0 = czero<double>(), h1 = czero<double>(), h2 = czero<double>();
-            if (hok[r]) {
-                c64 ax = hax[r], ay = hay[r];
+            bool okh = hok[r];
+            c64 ax = hax[r], ay = hay[r];
+            if (OFFD) {
+                const int ri = i0 - 1 + hi, rj = j0 - 1 + hj;
+                okh = ri < g.N[0] && rj < g.N[1] && (ri >= 0 || g.bloch[0]) && (rj >= 0 || g.bloch[1]);
+                ax = Hx[hi];
+                ay = Hy[hj];
+            }
+            if (okh) {
                 c64 X0x = Xs[bk][0][hj][hi], X0y = Xs[bk][1][hj][hi], X0z = Xs[bk][2][hj][hi];
                 const c64 Xnx = slab ? pzn * X0x : Xs[bn][0][hj][hi];
                 const c64 Xny = slab ? pzn * X0y : Xs[bn][1][hj][hi];
@@ -499,6 +548,15 @@ __global__ __launch_bounds__(TXW * TY) void k_apply_tiled(DGrid<double> g, OpDat
                     H0[c] = Hs[c][ty + 1][tx + 1];
                     Hxm[c] = Hs[c][ty + 1][tx];
                     Hym[c] = Hs[c][ty][tx + 1];
+                }
+                if (OFFD) {   // the owner's 1D factors, from the block tables
+                    exo = Fx[0][tx];
+                    eyo = Fy[0][ty];
+                    tmx = Fx[2][tx];
+                    tmy = Fy[2][ty];
+                    T2xy[0] = Fx[3][tx] * Fy[4][ty];
+                    T2xy[1] = Fx[4][tx] * Fy[3][ty];
+                    T2xy[2] = Fx[4][tx] * Fy[4][ty];
                 }
                 const c64 ez = ez_k;
                 c64 cc[3];
