@@ -221,7 +221,7 @@ int mfd_op_set_graddiv(mfd_op *op, int enable);
 /* Tile-shape autotuner for the stencil (rows per block TY, z-planes per block
  * KZ, tile order: bit 0 set = each XCD gets a contiguous run of tiles, clear =
  * dispatch order; bit 1 set = 32-wide tiles of 2·TY rows, a candidate when
- * ε is diagonal): times each
+ * ε is diagonal and the x size is not a multiple of 64): times each
  * candidate inside complete solver iterations for nrhs
  * right-hand sides (iters of them, on the solver workspace) and keeps the one
  * with the fastest iteration per precision; ms[] is that iteration time.

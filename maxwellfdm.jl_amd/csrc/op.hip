@@ -1664,9 +1664,11 @@ int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best
     // solve (a tile order whose halo rows miss L2 costs the stencil and, through
     // the caches, the vector kernels around it).
     // 32-wide tiles (orders 2, 3) are candidates for the diagonal-ε stencil
-    // (they pay where 64-wide tiles would leave lanes of the last x tile idle)
+    // where 64-wide tiles would leave lanes of the last x tile idle (with an x
+    // size that is a multiple of 64 they only add halo columns: at 128³ they
+    // tie on the Krylov step within noise and lose 19% on the A apply)
     const int tys[4] = {1, 2, 4, 8}, kzs[4] = {4, 8, 16, 32};
-    const int norders = (!op->full && !op->gd) ? 4 : 2;
+    const int norders = (op->G.h.N[0] % 64 != 0 && !op->full && !op->gd) ? 4 : 2;
     double bestms[2] = {1e30, 1e30};
     int bt[2][3] = {{4, 16, 1}, {4, 16, 1}};
     const int its = std::max(2, std::min(iters, 50));
