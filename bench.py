@@ -24,10 +24,10 @@ sys.path.insert(0, os.path.join(ROOT, "maxwellfdm.jl_amd"))
 HBM_PEAK_GBPS = 8000.0
 METRIC = "solve time to rel. residual 1e-8 + curl-curl apply GB/s; mixed vs FP64 on GPU"
 KERNELS = ("stencil", "upd_r", "upd_dzq")
-KERNEL_NAMES = {  # device symbols as rocprofv3 lists them (prefix match)
-    "stencil": "k_apply_tiled<{R}, 1, 2",
-    "upd_r": "k_upd_r_dot<{R},",
-    "upd_dzq": "k_upd_dzq<{R},",
+KERNEL_NAMES = {  # device kernels as rocprofv3 lists them, template arguments abridged
+    "stencil": "k_apply_tiled<{R}, 1, 2, ...> (Krylov stencil, qᵀPq epilogue)",
+    "upd_r": "k_upd_r_dot<{R}, ...> (r update, α and ‖r‖², rᵀr)",
+    "upd_dzq": "k_upd_dzq<{R}, ...> (d, z, q update)",
 }
 
 
