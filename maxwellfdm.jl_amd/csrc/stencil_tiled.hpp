@@ -15,9 +15,9 @@
 // one-cell halo ring) in LDS, computes H = Cₑ X of plane k on the tile plus
 // its −x/−y halo in LDS, and forms y at plane k from H(k), its −x/−y
 // neighbours and H(k−1) (registers of the owning thread).  Every 1D metric
-// factor a thread needs (x from its column, y from its row) is loaded once
-// into registers; the z factors of a plane are block-uniform.  H is never
-// written to memory.
+// factor a thread needs (x from its column, y from its row) is loaded once,
+// into registers (or, in the full-ε form, into block tables); the z factors
+// of a plane are block-uniform.  H is never written to memory.
 //
 // GD adds the grad-div term of the augmented inner operator
 //     A' = Cₘ Cₑ − ω² ε − ε G W Div ε,
@@ -151,12 +151,12 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
     // w0 = ∆l_dual(i)/(2 ∆l_prim(i)) and w1 = ∆l_dual(i−1)/(2 ∆l_prim(i))
     __shared__ c64 Wx[2][OFFD ? C::TX : 1];
     __shared__ c64 Wy[2][OFFD ? TY : 1];
-    // OFFD: the owner's per-column and per-row 1D factors (1/∆l_p, 1/∆l_d, the
-    // axis term, ∆l_d, ∆l_p) and the H points' 1/∆l_d, read from block tables
-    // every plane instead of held in registers (the full-ε stencil is the one
-    // short of registers)
-    __shared__ c64 Fx[OFFD ? 5 : 1][OFFD ? C::TX : 1];
-    __shared__ c64 Fy[OFFD ? 5 : 1][OFFD ? TY : 1];
+    // OFFD: the owner's per-column and per-row 1D factors (1/∆l_p, the axis
+    // term, ∆l_d, ∆l_p) and the H points' 1/∆l_d, read from block tables every
+    // plane instead of held in registers (the full-ε stencil is the one short
+    // of registers)
+    __shared__ c64 Fx[OFFD ? 4 : 1][OFFD ? C::TX : 1];
+    __shared__ c64 Fy[OFFD ? 4 : 1][OFFD ? TY : 1];
     __shared__ c64 Hx[OFFD ? C::HW : 1];
     __shared__ c64 Hy[OFFD ? C::HHt : 1];
     // Tile of this block.  Workgroups are dispatched to the 8 XCDs round-robin
@@ -245,18 +245,16 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
         if (tid < C::TX) {
             const int ii = min(i0 + tid, g.N[0] - 1);
             Fx[0][tid] = g.sdlinv[PRIM][0][ii];
-            Fx[1][tid] = g.sdlinv[DUAL][0][ii];
-            Fx[2][tid] = axis_term(g, 0, ii);
-            Fx[3][tid] = g.sdl[DUAL][0][ii];
-            Fx[4][tid] = g.sdl[PRIM][0][ii];
+            Fx[1][tid] = axis_term(g, 0, ii);
+            Fx[2][tid] = g.sdl[DUAL][0][ii];
+            Fx[3][tid] = g.sdl[PRIM][0][ii];
         }
         if (tid < TY) {
             const int jj = min(j0 + tid, g.N[1] - 1);
             Fy[0][tid] = g.sdlinv[PRIM][1][jj];
-            Fy[1][tid] = g.sdlinv[DUAL][1][jj];
-            Fy[2][tid] = axis_term(g, 1, jj);
-            Fy[3][tid] = g.sdl[DUAL][1][jj];
-            Fy[4][tid] = g.sdl[PRIM][1][jj];
+            Fy[1][tid] = axis_term(g, 1, jj);
+            Fy[2][tid] = g.sdl[DUAL][1][jj];
+            Fy[3][tid] = g.sdl[PRIM][1][jj];
         }
         for (int t = tid; t < C::HW; t += C::NT) {   // HW = TX + 1 exceeds the 64 threads of TY = 1
             const int ri = i0 - 1 + t;
@@ -555,11 +553,11 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
                 if (OFFD) {   // the owner's 1D factors, from the block tables
                     exo = Fx[0][tx];
                     eyo = Fy[0][ty];
-                    tmx = Fx[2][tx];
-                    tmy = Fy[2][ty];
-                    T2xy[0] = Fx[3][tx] * Fy[4][ty];
-                    T2xy[1] = Fx[4][tx] * Fy[3][ty];
-                    T2xy[2] = Fx[4][tx] * Fy[4][ty];
+                    tmx = Fx[1][tx];
+                    tmy = Fy[1][ty];
+                    T2xy[0] = Fx[2][tx] * Fy[3][ty];
+                    T2xy[1] = Fx[3][tx] * Fy[2][ty];
+                    T2xy[2] = Fx[3][tx] * Fy[3][ty];
                 }
                 const c64 ez = ez_k;
                 c64 cc[3];
