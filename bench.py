@@ -265,7 +265,12 @@ def run_batched_rhs(a):
     # with the tile shape tuned for one RHS
     nsingle = a.nrhs if a.nrhs <= 8 else 1
     print(f"[batched-rhs] batched solve {t_batch:.2f} s; {nsingle} single-RHS solve(s)", file=sys.stderr, flush=True)
-    op.autotune(1, 5)
+    t1 = op.autotune(1, 5)
+    # the single-RHS stencil at this size: FP64 A apply (144 B/cell) and the
+    # Krylov step's kernels, with the tile tuned for one RHS
+    ms_apply = op.bench_apply(1, "fp64", iters=50)
+    it_one = op.bench_iteration(1, "fp64", 20)
+    cells = m.grid.M
     singles, errs = [], []
     for r in range(nsingle):
         e1 = torch.zeros_like(b_d[r])
@@ -282,7 +287,10 @@ def run_batched_rhs(a):
            "batched_s": round(t_batch, 3), "iterations_max": int(st.iterations),
            "iterations_fp32": int(st.iterations_fp32), "relres_max": st.relres,
            "single_rhs_s": [round(t, 3) for t, _ in singles], "single_rhs_iterations": [k for _, k in singles],
-           "s_per_rhs_batched": round(t_batch / a.nrhs, 3), "rel_diff_vs_single": errs}
+           "s_per_rhs_batched": round(t_batch / a.nrhs, 3), "rel_diff_vs_single": errs,
+           "single_rhs_tiles": [list(t1[0]), list(t1[1])], "fp64_A_apply_ms": round(ms_apply, 4),
+           "fp64_A_apply_GBps": round(144.0 * cells / (ms_apply * 1e-3) / 1e9, 1),
+           "iteration_kernels_ms_1rhs_fp64": [round(x, 4) for x in it_one]}
     if nsingle == a.nrhs:
         out["singles_total_s"] = round(sum(t for t, _ in singles), 3)
         out["speedup_batched_vs_singles"] = round(out["singles_total_s"] / t_batch, 3)
