@@ -175,3 +175,30 @@ def test_nan_rhs_reports_noconv(method, pkg, O):
     op.close()
     assert not st.converged and np.isnan(st.relres)
     assert np.linalg.norm(b - A @ E[0]) / np.linalg.norm(b) <= 1.05e-8
+
+
+def test_bicgstab_probes(pkg, O, capfd, monkeypatch):
+    """BiCGSTAB cycles probe the true FP64 residual every 2,048 steps: on the
+    64³ random-box system (thousands of BiCGSTAB steps) the probes run, the
+    cycle ends on a probe that meets the tolerance, and the returned iterate is
+    no worse than any probe and meets the tolerance on the oracle's A."""
+    import re
+    from maxwellfdfd import workloads as W
+    m, om = W.random_boxes(N=64, npml=10, nbox=40, lam=24.0, seed=1, ndip=1)
+    pkg.calc_matparams(m)
+    je, jm = pkg.create_srcs(m)
+    b = pkg.create_b(pkg.EE, om, m, je, jm)
+    A = O.to_scipy(O.create_A(spec_of(m), om, m.εarr, m.μarr))
+    monkeypatch.setenv("MFD_SOLVE_TRACE", "1")
+    op = pkg.MaxwellOperator(m, om)
+    say("bicgstab probes")
+    e, st = op.solve(b, tol=1e-8, maxit=60000, method="bicgstab", raise_on_noconv=False)
+    op.close()
+    err = capfd.readouterr().err
+    probes = [float(v) for v in re.findall(r"\[mfd probe\] it \d+ true residual ([0-9.e+-]+)", err)]
+    rel = np.linalg.norm(b - A @ e) / np.linalg.norm(b)
+    say("  iterations", st.iterations, "probes", len(probes), probes[-3:], "returned", rel, "relres", st.relres)
+    assert st.iterations > 4096, "premise: a solve long enough for probes"
+    assert probes, "no probe ran"
+    assert st.converged and rel <= 1.05e-8
+    assert st.relres <= min(probes) * (1 + 1e-9)
