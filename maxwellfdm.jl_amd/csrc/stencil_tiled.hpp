@@ -132,7 +132,8 @@ __device__ inline void plane_init(const DGrid<R> &g, int vk, PlaneZ<R> &z) {
 // OFFD: the ε tensor has corner off-diagonals (xy, xz, yz); their term is
 // formed from the staged planes k−1 (three X_z values kept in registers),
 // k and k+1, with the corner ε read from global memory.
-template <typename R, int MODE, int EPI, int TY, bool GD, bool FUSE = false, bool OFFD = false, int TXW = 64>
+template <typename R, int MODE, int EPI, int TY, bool GD, bool FUSE = false, bool OFFD = false, int TXW = 64,
+          bool MU = false>
 __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<double> g, OpData<double> od, c64 om2,
                                                          const cx<R> *__restrict__ xall,
                                                          cx<R> *__restrict__ yall, const c64 *__restrict__ ball,
@@ -283,6 +284,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
     constexpr int NHP = (C::HHt * C::HW + C::NT - 1) / C::NT;
     c64 hax[NHP], hay[NHP];
     bool hok[NHP];
+    int64_t hxy[MU ? NHP : 1];   // MU: wrapped x-y cell of each H point
 #pragma unroll
     for (int r = 0; r < NHP; ++r) {
         hok[r] = false;
@@ -295,6 +297,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
         int ix = ri < 0 ? g.N[0] - 1 : (ri < g.N[0] ? ri : 0), iy = rj < 0 ? g.N[1] - 1 : (rj < g.N[1] ? rj : 0);
         hax[r] = g.sdlinv[DUAL][0][ix];
         hay[r] = g.sdlinv[DUAL][1][iy];
+        if (MU) hxy[r] = ix + (int64_t)g.N[0] * iy;
     }
     // ψ nodes of this thread: LDS (pi, pj) -> global (i0+pi, j0+pj), pi in [0,64], pj in [0,TY]
     constexpr int NPP = GD ? (C::PHt * C::PW + C::NT - 1) / C::NT : 1;
@@ -350,7 +353,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
                         v = xv * qv;
                     } else v = xv;
                     c64 ev = cone<double>();
-                    if (GD || (MODE == MODE_A && s == 0)) ev = cvt<double>(od.epsd[3 * p + c]);
+                    if (GD || ((MODE == MODE_A || MU) && s == 0)) ev = cvt<double>(od.epsd[3 * p + c]);
                     if (GD) yv = ev * v;
                     if (s == 0) { xr[c] = xv; qr[c] = qv; er[c] = ev; }
                     v = v * ph;
@@ -384,7 +387,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
             for (int c = 0; c < 3; ++c) {
                 pfx[s][c] = ok ? x[dofidx(g, c, p)] : mk<R>((R)0, (R)0);
                 if (MODE == MODE_P) pfq[s][c] = ok ? od.qs[3 * p + c] : czero<double>();
-                if (MODE == MODE_A && s == 0) pfe[c] = ok ? od.epsd[3 * p + c] : cone<double>();
+                if ((MODE == MODE_A || MU) && s == 0) pfe[c] = ok ? od.epsd[3 * p + c] : cone<double>();
             }
         }
     };
@@ -403,7 +406,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
                 if (s == 0) {
                     xr[c] = xv;
                     qr[c] = MODE == MODE_P ? pfq[s][c] : cone<double>();
-                    er[c] = MODE == MODE_A ? pfe[c] : cone<double>();
+                    er[c] = (MODE == MODE_A || MU) ? pfe[c] : cone<double>();
                 }
                 Xs[b][c][sjj[s]][sii[s]] = v * ph;
             }
@@ -412,6 +415,19 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
 
     // H of an interior plane (buffers bk: that plane, bn: the next) for all H
     // points; az is 1/s∆l_z of the plane
+    // MU: 1/μ at this thread's H points of the plane being processed, loaded at
+    // the top of the plane (ahead of the prefetch, so waiting for them does
+    // not wait for it)
+    c64 hmu[MU ? NHP : 1][3];
+    auto load_mu = [&](int vk) {
+        PlaneZ<double> z;
+        plane_init(g, vk, z);
+#pragma unroll
+        for (int r = 0; r < (MU ? NHP : 0); ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                hmu[r][c] = hok[r] && z.ok ? od.muinv[3 * (hxy[r] + NxNy * z.gk) + c] : czero<double>();
+    };
     // slab: a one-cell z axis, whose plane k+1 is plane k times pzn (the
     // Bloch phase, or 0 past a non-Bloch end) and is not staged
     auto hplane = [&](int bk, int bn, c64 az, bool slab, c64 pzn) {
@@ -436,6 +452,11 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
                 h0 = ay * (Xs[bk][2][hj + 1][hi] - X0z) - az * (Xny - X0y);
                 h1 = az * (Xnx - X0x) - ax * (Xs[bk][2][hj][hi + 1] - X0z);
                 h2 = ax * (Xs[bk][1][hj][hi + 1] - X0y) - ay * (Xs[bk][0][hj + 1][hi] - X0x);
+            }
+            if (MU) {   // H = μ⁻¹ Cₑ X (zero points stay zero)
+                h0 = h0 * hmu[r][0];
+                h1 = h1 * hmu[r][1];
+                h2 = h2 * hmu[r][2];
             }
             Hs[0][hj][hi] = h0;
             Hs[1][hj][hi] = h1;
@@ -481,10 +502,16 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
         // prefetch, as every per-thread load in the pipelined loop is)
         const bool hpv = own && (k0 > 0 || g.bloch[2]);
         c64 pax = czero<double>(), pay = czero<double>(), paz = czero<double>();
+        c64 pmx = cone<double>(), pmy = cone<double>();   // MU: 1/μ of H_x, H_y at plane k0−1
         if (hpv) {
             paz = g.sdlinv[DUAL][2][k0 > 0 ? k0 - 1 : g.N[2] - 1];
             pax = g.sdlinv[DUAL][0][i];
             pay = g.sdlinv[DUAL][1][j];
+            if (MU) {
+                const int64_t pm1 = i + (int64_t)g.N[0] * j + NxNy * (k0 > 0 ? k0 - 1 : g.N[2] - 1);
+                pmx = od.muinv[3 * pm1 + 0];
+                pmy = od.muinv[3 * pm1 + 1];
+            }
         }
         if (slab) {   // one plane, staged once into buffer 1
             fetch(0);
@@ -513,6 +540,10 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
             hpx = pay * (Xs[0][2][hj + 1][hi] - X0z) - paz * (Xs[1][1][hj][hi] - X0y);
             hpy = paz * (Xs[1][0][hj][hi] - X0x) - pax * (Xs[0][2][hj][hi + 1] - X0z);
         }
+        if (MU) {
+            hpx = hpx * pmx;
+            hpy = hpy * pmy;
+        }
         if (GD) pplane(1, 0, k0, k0 & 1);
         if (OFFD && own) {   // X_z of plane k0−1 at (i, j), (i+1, j), (i, j+1)
             zkm0 = Xs[0][2][ty + 1][tx + 1];
@@ -527,6 +558,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
             // barrier that published Hs, so only the prologue's reads of the
             // first two planes need this one
             if (GD || OFFD || k == k0) __syncthreads();
+            if (MU) load_mu(k);
             // z factors of plane k (scalar loads: waiting for them does not
             // wait for the prefetch)
             const c64 az_k = ldu(g.sdlinv[DUAL][2], k), ez_k = ldu(g.sdlinv[PRIM][2], k);
@@ -633,14 +665,14 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
                         const c64 w = qc[u] * (T2xy[u] * (u == 2 ? sdD : sdP));
                         const c64 v = xc[u] * qc[u];
                         c64 in;
-                        if (GD) in = cc[u] - om2 * (ec[u] * v);
+                        if (GD || MU) in = cc[u] - om2 * (ec[u] * v);
                         else {
                             const c64 cd = u == 0 ? tmy + tzm : (u == 1 ? tzm + tmx : tmx + tmy);
                             in = cc[u] - cd * v;
                         }
                         if (OFFD) in = in - om2 * term[u];
                         out = w * in;
-                        if (!GD) out = out + xc[u];
+                        if (!GD && !MU) out = out + xc[u];
                     }
                     int64_t e = dofidx(g, u, p);
                     if (EPI == EPI_RESID) {

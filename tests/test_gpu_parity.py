@@ -232,17 +232,18 @@ OPCASES = ["uniform_pml", "nonuniform_bloch", "mixed_pml_bloch", "tiny_bloch_axe
 @pytest.mark.parametrize("ty,kz,xo", [(1, 1, 1), (1, 4, 0), (2, 3, 1), (2, 16, 0), (4, 2, 1), (4, 16, 0),
                                       (4, 16, 1), (8, 3, 1), (8, 16, 0), (0, 1, 0),
                                       (1, 3, 3), (2, 5, 2), (4, 16, 3), (8, 2, 3)])
-@pytest.mark.parametrize("full", [False, True])
-def test_apply_tile_shapes(case, ty, kz, xo, full, pkg, O):
+@pytest.mark.parametrize("params", ["diag", "full", "mu"])
+def test_apply_tile_shapes(case, ty, kz, xo, params, pkg, O):
     """Every tile shape and tile order of the stencil (and the direct kernel,
-    TY = 0) gives the CSC product, with diagonal and with full ε (the corner
-    off-diagonal terms read planes k−1, k, k+1 and wrap with the Bloch phase).
-    Orders 2 and 3 ask for 32-wide tiles, which the diagonal-ε stencil uses
-    and the full-ε one ignores."""
+    TY = 0) gives the CSC product, with diagonal ε, with full ε (the corner
+    off-diagonal terms read planes k−1, k, k+1 and wrap with the Bloch phase)
+    and with μ ≠ 1 (1/μ weights on the H plane).  Orders 2 and 3 ask for
+    32-wide tiles, which the diagonal-ε, μ = 1 stencil uses and the others
+    ignore."""
     m = make(pkg, case)
-    eps, muu = rand_params(m, 8, full=full)
+    eps, muu = rand_params(m, 8, full=params == "full", mu=params == "mu")
     om = 0.55
-    say("tiles", case, ty, kz, xo, full)
+    say("tiles", case, ty, kz, xo, params)
     A = O.to_scipy(O.create_A(spec_of(m), om, eps, muu))
     op = pkg.MaxwellOperator(m, om, εarr=eps, μarr=muu)
     op.set_tile("fp64", ty, kz, xo)
