@@ -191,8 +191,9 @@ int mfd_create_b(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const m
  * resident.  eps_arr/mu_arr as produced by create_paramops (host).
  * ft = EE: A = Cₘ (Pμ \ Cₑ) − ω² Pε; ft = HH: A = Cₑ (Pε \ Cₘ) − ω² Pμ, built as
  * the identical EE operator of the same grid with every boundft flipped and
- * ε, μ swapped.  Any boundft; the tiled stencils serve boundft = EE on every
- * axis with μ = 1 (EE) — other operators use the direct kernel.  Off-diagonal
+ * ε, μ swapped.  Any boundft; the tiled stencils serve every operator but
+ * one with both μ ≠ 1 and off-diagonal ε, which uses the direct kernel
+ * (boundft = HH axes run on the grid reflected along them).  Off-diagonal
  * ε (EE) needs boundft = EE on every axis, off-diagonal μ (HH) boundft = HH. */
 int mfd_op_create(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const mfd_c64 *eps_arr,
                   const mfd_c64 *mu_arr, mfd_op **out);

@@ -82,6 +82,9 @@ struct DGrid {
     // operator built from an assembled matrix); null: T from the tables above
     const cx<R> *tdof = nullptr;
     const cx<R> *tdofinv = nullptr;
+    // axes along which this view is the reflection of the operator's grid
+    // (bit a: index i of axis a is the caller's N_a−1−i); 0 for the grid itself
+    int rmask = 0;
 };
 
 __host__ __device__ inline int g0type(const int *bft, int ft, int a) {

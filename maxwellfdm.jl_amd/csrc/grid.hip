@@ -140,6 +140,70 @@ DevGrid::~DevGrid() { release(); }
 void DevGrid::release() {
     if (buf) hipFree(buf);
     buf = nullptr;
+    if (rbuf) hipFree(rbuf);
+    rbuf = nullptr;
+}
+
+// reflected tables: dst[t][w][i] = src[1−t][w][N_w−1−i] for the four kinds
+// along the axes of mask, dst = src along the others
+struct ReflectArgs {
+    int64_t off[2][3], N[3];
+    int mask;
+};
+__global__ void k_reflect_tables(ReflectArgs a, int64_t tot, const c64 *s0, const c64 *s1, const c64 *s2,
+                                 const c64 *s3, c64 *d0, c64 *d1, c64 *d2, c64 *d3) {
+    int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= tot) return;
+    int t = 0, w = 0;
+    for (int tt = 0; tt < 2; ++tt)
+        for (int ww = 0; ww < 3; ++ww)
+            if (e >= a.off[tt][ww] && e < a.off[tt][ww] + a.N[ww]) { t = tt; w = ww; }
+    const int64_t i = e - a.off[t][w];
+    const int64_t src = ((a.mask >> w) & 1) ? a.off[1 - t][w] + (a.N[w] - 1 - i) : e;
+    d0[e] = s0[src];
+    d1[e] = s1[src];
+    d2[e] = s2[src];
+    d3[e] = s3[src];
+}
+
+int DevGrid::reflect(int mask, hipStream_t stream) {
+    if (rbuf) return MFD_OK;
+    MFD_HIP(hipMalloc(&rbuf, sizeof(c64) * 4 * total));
+    c64 *dst[4];
+    for (int k = 0; k < 4; ++k) dst[k] = (c64 *)rbuf + k * total;
+    ReflectArgs a;
+    int64_t off = 0;
+    for (int t = 0; t < 2; ++t)
+        for (int w = 0; w < 3; ++w) {
+            a.off[t][w] = off;
+            off += h.N[w];
+        }
+    for (int w = 0; w < 3; ++w) a.N[w] = h.N[w];
+    a.mask = mask;
+    // the four kinds of d64, in the order fill_view lays them out
+    const c64 *src[4] = {d64.sdl[PRIM][0], d64.sdlinv[PRIM][0], d64.ssq[PRIM][0], d64.ssqinv[PRIM][0]};
+    const int bs = 256;
+    k_reflect_tables<<<(unsigned)((total + bs - 1) / bs), bs, 0, stream>>>(a, total, src[0], src[1], src[2], src[3],
+                                                                           dst[0], dst[1], dst[2], dst[3]);
+    MFD_HIP(hipGetLastError());
+    r64 = d64;
+    for (int t = 0; t < 2; ++t)
+        for (int w = 0; w < 3; ++w) {
+            r64.sdl[t][w] = dst[0] + a.off[t][w];
+            r64.sdlinv[t][w] = dst[1] + a.off[t][w];
+            r64.ssq[t][w] = dst[2] + a.off[t][w];
+            r64.ssqinv[t][w] = dst[3] + a.off[t][w];
+        }
+    for (int w = 0; w < 3; ++w) {
+        r64.bft[w] = EE;
+        if ((mask >> w) & 1) {
+            r64.ph[w] = d64.phinv[w];
+            r64.phinv[w] = d64.ph[w];
+        }
+    }
+    r64.tdof = r64.tdofinv = nullptr;
+    r64.rmask = mask;
+    return MFD_OK;
 }
 
 int DevGrid::create(const mfd_grid *g, hipStream_t stream) {

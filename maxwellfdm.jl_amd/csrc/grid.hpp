@@ -13,7 +13,19 @@ struct DevGrid {
     const c64 *sdl64 = nullptr;   // [gt][axis][N] stretched ∆l, complex double
     DGrid<double> d64;
     DGrid<float> d32;
+    // The grid seen through the reflection i -> N_a−1−i of the axes in mask:
+    // along those axes every 1D table reversed with its primal and dual types
+    // swapped and the Bloch phase inverted; every boundft EE.  An operator
+    // whose boundft is HH on exactly those axes is, in these coordinates and
+    // with the E components along the reflected axes negated (E is a polar
+    // vector), the all-EE operator: along a reflected axis each curl's
+    // difference direction and ghost side flip, and the sign each derivative
+    // picks up is absorbed by those components and by H (an axial vector).
+    // Built by reflect(); null tables until then.
+    DGrid<double> r64;
+    void *rbuf = nullptr;
     int create(const mfd_grid *g, hipStream_t stream);
+    int reflect(int mask, hipStream_t stream);
     void release();
     ~DevGrid();
     template <typename R> const DGrid<R> &view() const;

@@ -1016,7 +1016,7 @@ void Workspace::release() {
 // the tiled stencils do not serve); allocated outside any graph capture
 static int hbuf_ensure(mfd_op *op, int64_t nrhs) {
     if (op->csr) return MFD_OK;
-    const bool tiled_ok = !(op->has_mu && op->full) && op->ee_all;
+    const bool tiled_ok = !(op->has_mu && op->full) && (op->ee_all || op->rev);
     if (tiled_ok && op->tile64[0] > 0 && op->tile32[0] > 0) return MFD_OK;
     const size_t need = sizeof(c64) * 3 * (size_t)op->G.h.M * (size_t)nrhs;
     if (op->hbuf_bytes >= need) return MFD_OK;
@@ -1129,17 +1129,28 @@ static int launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int6
     }
     if (MODE == MODE_P && op->gd && ty == 8) ty = 4;   // the grad-div tile keeps two more LDS planes
     // μ ≠ 1 has a tiled form with diagonal ε (the grad-div form is never on with μ)
-    if (op->ee_all && ty > 0 && !(op->has_mu && op->full)) {
+    if ((op->ee_all || op->rev) && ty > 0 && !(op->has_mu && op->full)) {
         // order bit 1: 32-wide tiles of 2·TY rows (same threads per block);
-        // the off-diagonal, grad-div and μ stencils are built 64-wide only
-        const bool narrow = (xr & 2) && !op->full && !(MODE == MODE_P && op->gd) && !op->has_mu;
+        // the off-diagonal, grad-div, μ and reflected stencils are built 64-wide only
+        const bool narrow = (xr & 2) && !op->full && !(MODE == MODE_P && op->gd) && !op->has_mu && !op->rev;
         xr &= 1;
         const int tw = narrow ? 32 : 64, rows = narrow ? 2 * ty : ty;
         int nx = (g.N[0] + tw - 1) / tw, ny = (g.N[1] + rows - 1) / rows;
         int nch = (g.N[2] + kz - 1) / kz;
         while ((int64_t)nx * ny * nch > NPART) { kz *= 2; nch = (g.N[2] + kz - 1) / kz; }
         dim3 grid(nx, ny, (unsigned)(nch * nrhs));
-        if (op->full) {   // off-diagonal ε (grad-div is never enabled with it)
+        const DGrid<double> &gr = op->G.r64;
+        if (op->rev && op->has_mu) {   // boundft HH on some axis: the grid reflected along those axes
+            if (ty == 8) k_apply_tiled<R, MODE, EPI, 8, false, false, false, 64, true, true><<<grid, 512, 0, s>>>(gr, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, false, false, false, 64, true, true><<<grid, 128, 0, s>>>(gr, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, false, false, false, 64, true, true><<<grid, 256, 0, s>>>(gr, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else k_apply_tiled<R, MODE, EPI, 1, false, false, false, 64, true, true><<<grid, 64, 0, s>>>(gr, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+        } else if (op->rev) {
+            if (ty == 8) k_apply_tiled<R, MODE, EPI, 8, false, false, false, 64, false, true><<<grid, 512, 0, s>>>(gr, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, false, false, false, 64, false, true><<<grid, 128, 0, s>>>(gr, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, false, false, false, 64, false, true><<<grid, 256, 0, s>>>(gr, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else k_apply_tiled<R, MODE, EPI, 1, false, false, false, 64, false, true><<<grid, 64, 0, s>>>(gr, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+        } else if (op->full) {   // off-diagonal ε (grad-div is never enabled with it)
             if (ty == 8) k_apply_tiled<R, MODE, EPI, 8, false, false, true><<<grid, 512, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
             else if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, false, false, true><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
             else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, false, false, true><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
@@ -1267,8 +1278,12 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
     op->stream = ctx->stream;
     op->ft = ft;
     for (int a = 0; a < 3; ++a)
-        if (g->boundft[a] != MFD_EE) op->ee_all = 0;
+        if (g->boundft[a] != MFD_EE) {
+            op->ee_all = 0;
+            op->rev |= 1 << a;
+        }
     int rc = op->G.create(g, ctx->stream);
+    if (!rc && op->rev) rc = op->G.reflect(op->rev, ctx->stream);
     if (rc) { delete op; return rc; }
     hipStream_t s = ctx->stream;
     int64_t M = op->G.h.M;
@@ -1339,14 +1354,14 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
     for (int a = 0; a < 3; ++a)
         if (op->G.d64.bloch[a] && fabs(op->G.d64.ph[a].im) > 1e-14) op->sym = 0;
     op->gd_ok = !full && !op->has_mu && op->ee_all && !(om2.re == 0.0 && om2.im == 0.0);
-    if (!op->ee_all) op->tile64[0] = op->tile32[0] = 0;   // the direct kernel serves other boundaries
+    if (!op->ee_all && !op->rev) op->tile64[0] = op->tile32[0] = 0;   // the direct kernel serves mixed boundaries
     else {
         // default tile (until mfd_op_autotune or mfd_op_set_tile): 4 rows x 16
         // planes, with fewer planes and then fewer rows per block on grids too
         // small for that to give one tile per CU (256); 32-wide tiles (of
         // twice the rows) where they leave fewer x lanes idle than 64-wide ones
         const int64_t N0 = op->G.h.N[0];
-        const bool narrow = !full && !op->has_mu && 64 * ((N0 + 63) / 64) - N0 > 32 * ((N0 + 31) / 32) - N0;
+        const bool narrow = !full && !op->has_mu && !op->rev && 64 * ((N0 + 63) / 64) - N0 > 32 * ((N0 + 31) / 32) - N0;
         const int64_t nx = narrow ? (N0 + 31) / 32 : (N0 + 63) / 64;
         auto tiles = [&](int ty, int kz) {
             const int rows = narrow ? 2 * ty : ty;
@@ -1658,7 +1673,7 @@ int op_bench_iteration_impl(mfd_op *op, int64_t nrhs, int prec, int iters, doubl
 // order) and keeps the fastest per precision.
 int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best32, double *ms_out) {
     if (op->csr) { set_error("autotune: an operator built from a CSC matrix has no stencil tiles"); return MFD_ERR_UNSUPPORTED; }
-    if ((op->has_mu && op->full) || !op->ee_all) {   // the direct kernel serves these operators
+    if ((op->has_mu && op->full) || !(op->ee_all || op->rev)) {   // the direct kernel serves these operators
         op->tile64[0] = op->tile32[0] = 0;
         if (best64) best64[0] = best64[1] = best64[2] = 0;
         if (best32) best32[0] = best32[1] = best32[2] = 0;
@@ -1674,7 +1689,7 @@ int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best
     // size that is a multiple of 64 they only add halo columns: at 128³ they
     // tie on the Krylov step within noise and lose 19% on the A apply)
     const int tys[4] = {1, 2, 4, 8}, kzs[4] = {4, 8, 16, 32};
-    const int norders = (op->G.h.N[0] % 64 != 0 && !op->full && !op->gd && !op->has_mu) ? 4 : 2;
+    const int norders = (op->G.h.N[0] % 64 != 0 && !op->full && !op->gd && !op->has_mu && !op->rev) ? 4 : 2;
     double bestms[2] = {1e30, 1e30};
     int bt[2][3] = {{4, 16, 1}, {4, 16, 1}};
     const int its = std::max(2, std::min(iters, 50));

@@ -87,6 +87,7 @@ struct mfd_op {
     void *hbuf = nullptr;     // H = μ⁻¹ Cₑ x scratch of the two-pass direct kernel (3M values per RHS)
     size_t hbuf_bytes = 0;
     int ee_all = 1;           // boundft = EE on every axis (the tiled stencils and field conversions need it)
+    int rev = 0;              // axes with boundft = HH (bit a): the tiled stencil runs on G.r64, reflected along them
     // an operator built from a caller's CSC matrix A (mfd_op_create_csc):
     // row-major copy of A, applied by a sparse product; its grid is a 1D unit
     // grid of n/3 cells that only provides the DOF numbering (T = 1)

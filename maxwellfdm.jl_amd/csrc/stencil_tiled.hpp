@@ -132,8 +132,14 @@ __device__ inline void plane_init(const DGrid<R> &g, int vk, PlaneZ<R> &z) {
 // OFFD: the ε tensor has corner off-diagonals (xy, xz, yz); their term is
 // formed from the staged planes k−1 (three X_z values kept in registers),
 // k and k+1, with the corner ε read from global memory.
+// REV: the grid g is the reflection of the operator's grid along the axes of
+// g.rmask (boundft HH there, EE in g) and the kernel works in its
+// coordinates: every per-cell global access (vectors, qs, ε, 1/μ) goes to the
+// caller's cell (index N_a−1−i along a reflected axis), so the vectors keep
+// their order, and the E components along reflected axes are negated on
+// load and on store.
 template <typename R, int MODE, int EPI, int TY, bool GD, bool FUSE = false, bool OFFD = false, int TXW = 64,
-          bool MU = false>
+          bool MU = false, bool REV = false>
 __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<double> g, OpData<double> od, c64 om2,
                                                          const cx<R> *__restrict__ xall,
                                                          cx<R> *__restrict__ yall, const c64 *__restrict__ ball,
@@ -143,6 +149,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
                                                          cx<R> *__restrict__ pnall = nullptr,
                                                          cx<R> *__restrict__ itall = nullptr) {
     using C = TileCfg<TY, TXW>;
+    static_assert(!(REV && (GD || OFFD || FUSE)), "the reflected form serves diagonal ε, unfused, without grad-div");
     constexpr int GB = GD ? 2 : 1, GH = GD ? C::XH : 1, GW = GD ? C::XW : 1;
     __shared__ c64 Xs[2][3][C::XH][C::XW];
     __shared__ c64 Ys[GB][3][GH][GW];
@@ -201,6 +208,15 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
     const int k0 = chunk * KZ, k1 = min(k0 + KZ, g.N[2]);
     const int64_t NxNy = (int64_t)g.N[0] * g.N[1];
     double acc[2] = {0.0, 0.0};
+    // the caller's cell of cell (ci, cj, ck) of g: x-y part and z part
+    auto gxy = [&](int ci, int cj) -> int64_t {
+        if (REV && (g.rmask & 1)) ci = g.N[0] - 1 - ci;
+        if (REV && (g.rmask & 2)) cj = g.N[1] - 1 - cj;
+        return ci + (int64_t)g.N[0] * cj;
+    };
+    auto gz = [&](int ck) -> int64_t { return NxNy * (REV && (g.rmask & 4) ? g.N[2] - 1 - ck : ck); };
+    // REV: E_c is negated along a reflected axis c
+    auto rsg = [&](int c, c64 v) -> c64 { return REV && ((g.rmask >> c) & 1) ? -v : v; };
 
     // staged points of this thread: its interior point (slot 0) and its share
     // of the halo ring (slots 1..NRS; the ring has 2·XW + 2·TY points)
@@ -297,7 +313,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
         int ix = ri < 0 ? g.N[0] - 1 : (ri < g.N[0] ? ri : 0), iy = rj < 0 ? g.N[1] - 1 : (rj < g.N[1] ? rj : 0);
         hax[r] = g.sdlinv[DUAL][0][ix];
         hay[r] = g.sdlinv[DUAL][1][iy];
-        if (MU) hxy[r] = ix + (int64_t)g.N[0] * iy;
+        if (MU) hxy[r] = gxy(ix, iy);
     }
     // ψ nodes of this thread: LDS (pi, pj) -> global (i0+pi, j0+pj), pi in [0,64], pj in [0,TY]
     constexpr int NPP = GD ? (C::PHt * C::PW + C::NT - 1) / C::NT : 1;
@@ -329,7 +345,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
             if (!sact[s]) continue;
             const StagePt<double> &P = sp[s];
             bool ok = P.ok && z.ok;
-            int64_t p = P.gi + (int64_t)g.N[0] * P.gj + NxNy * z.gk;
+            const int64_t p = gxy(P.gi, P.gj) + gz(z.gk);
             c64 ph = P.ph * z.ph;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -346,7 +362,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
                             pn[dof] = cvt<R>(xv);
                             itv[dof] = cvt<R>(cvt<double>(itv[dof]) + fapend * po);
                         }
-                    } else xv = cvt<double>(x[dof]);
+                    } else xv = rsg(c, cvt<double>(x[dof]));
                     c64 qv = cone<double>();
                     if (MODE == MODE_P) {
                         qv = od.qs[3 * p + c];   // q·T⁻¹
@@ -382,7 +398,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
         for (int s = 0; s < NPF; ++s) {
             if (!sact[s]) continue;
             const bool ok = sp[s].ok && z.ok;
-            const int64_t p = sp[s].gi + (int64_t)g.N[0] * sp[s].gj + NxNy * z.gk;
+            const int64_t p = gxy(sp[s].gi, sp[s].gj) + gz(z.gk);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 pfx[s][c] = ok ? x[dofidx(g, c, p)] : mk<R>((R)0, (R)0);
@@ -400,7 +416,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
             const c64 ph = sp[s].ph * z.ph;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const c64 xv = cvt<double>(pfx[s][c]);
+                const c64 xv = rsg(c, cvt<double>(pfx[s][c]));
                 c64 v = xv;
                 if (MODE == MODE_P) v = xv * pfq[s][c];
                 if (s == 0) {
@@ -426,7 +442,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
         for (int r = 0; r < (MU ? NHP : 0); ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                hmu[r][c] = hok[r] && z.ok ? od.muinv[3 * (hxy[r] + NxNy * z.gk) + c] : czero<double>();
+                hmu[r][c] = hok[r] && z.ok ? od.muinv[3 * (hxy[r] + gz(z.gk)) + c] : czero<double>();
     };
     // slab: a one-cell z axis, whose plane k+1 is plane k times pzn (the
     // Bloch phase, or 0 past a non-Bloch end) and is not staged
@@ -508,7 +524,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
             pax = g.sdlinv[DUAL][0][i];
             pay = g.sdlinv[DUAL][1][j];
             if (MU) {
-                const int64_t pm1 = i + (int64_t)g.N[0] * j + NxNy * (k0 > 0 ? k0 - 1 : g.N[2] - 1);
+                const int64_t pm1 = gxy(i, j) + gz(k0 > 0 ? k0 - 1 : g.N[2] - 1);
                 pmx = od.muinv[3 * pm1 + 0];
                 pmy = od.muinv[3 * pm1 + 1];
             }
@@ -674,14 +690,16 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
                         out = w * in;
                         if (!GD && !MU) out = out + xc[u];
                     }
-                    int64_t e = dofidx(g, u, p);
+                    int64_t e = dofidx(g, u, REV ? gxy(i, j) + gz(k) : p);
                     if (EPI == EPI_RESID) {
+                        const c64 oc = rsg(u, out);   // in the caller's frame
                         c64 bb = ball[rhs * vs + e];
-                        c64 rr = mk<double>(bb.re - out.re, bb.im - out.im);
+                        c64 rr = mk<double>(bb.re - oc.re, bb.im - oc.im);
                         if (yall) y[e] = cvt<R>(rr);   // null output: the residual norm only
                         acc[0] += (double)rr.re * rr.re + (double)rr.im * rr.im;
                     } else {
-                        y[e] = cvt<R>(out);
+                        // the dot product xᵀy is the same in either frame
+                        y[e] = cvt<R>(rsg(u, out));
                         if (EPI == EPI_DOT) {
                             acc[0] += (double)xc[u].re * out.re - (double)xc[u].im * out.im;
                             acc[1] += (double)xc[u].re * out.im + (double)xc[u].im * out.re;

@@ -257,6 +257,90 @@ def test_apply_tile_shapes(case, ty, kz, xo, params, pkg, O):
     op.close()
 
 
+# boundft = HH on some axes (an EE model so configured, or the HH formulation,
+# which flips every axis): the tiled stencil on the grid reflected along them
+REVCASES = {
+    "rev_pml": dict(N=(6, 7, 5), npml=((2, 1, 2), (1, 2, 1)), om=0.7, bloch=(False,) * 3, nu=5),
+    "rev_bloch": dict(N=(5, 4, 6), npml=((0,) * 3, (0,) * 3), om=0.9, bloch=(True,) * 3,
+                      kb=(0.3, -0.2, 0.5), nu=1),
+    "rev_mixed": dict(N=(4, 5, 3), npml=((1, 0, 1), (1, 1, 0)), om=0.6 + 0.05j, bloch=(True, False, True),
+                      kb=(0.1, 0.0, 0.4), nu=2),
+    "rev_wide": dict(N=(70, 5, 7), npml=((3, 1, 1), (2, 1, 2)), om=0.45, bloch=(False,) * 3, nu=6),
+    "rev_cmplast": dict(N=(4, 3, 5), npml=((1, 1, 1), (1, 1, 1)), om=0.8, bloch=(False,) * 3, cmpfirst=False),
+    "rev_slab": dict(N=(9, 7, 1), npml=((2, 1, 0), (1, 2, 0)), om=0.6, bloch=(False, True, True),
+                     kb=(0.0, 0.2, 0.35)),
+    "rev_tiny": dict(N=(1, 2, 5), npml=((0, 0, 1), (0, 0, 1)), om=0.5, bloch=(True, True, False),
+                     kb=(0.0, 0.3, 0.0)),
+}
+
+
+def make_rev(pkg, case, bft):
+    """The model of REVCASES[case] with the given boundft per axis."""
+    c = dict(REVCASES[case])
+    c["bft"] = bft
+    CASES["_rev"] = c
+    try:
+        return make(pkg, "_rev")
+    finally:
+        del CASES["_rev"]
+
+
+@pytest.mark.parametrize("case", list(REVCASES))
+@pytest.mark.parametrize("ty,kz,xo", [(1, 1, 1), (1, 4, 0), (2, 3, 1), (4, 2, 1), (4, 16, 0), (8, 3, 1),
+                                      (8, 16, 0), (0, 1, 0)])
+@pytest.mark.parametrize("ft,bft", [(0, (1, 1, 1)), (0, (0, 1, 0)), (0, (1, 0, 1)), (0, (0, 0, 1)),
+                                    (1, (0, 0, 0)), (1, (1, 0, 0))])
+@pytest.mark.parametrize("mu", [False, True])
+def test_apply_tile_shapes_reflected(case, ty, kz, xo, ft, bft, mu, pkg, O):
+    """boundft = HH on some or all axes, in the EE formulation and in the HH
+    formulation (which flips every axis): every tile shape of the stencil on
+    the reflected grid (and the direct kernel, TY = 0) against the oracle's
+    CSC of that formulation, with and without μ ≠ 1 (in the HH formulation ε
+    and μ trade places, so ε ≠ 1 there is the stencil's 1/μ weight)."""
+    m = make_rev(pkg, case, bft)
+    eps, muu = rand_params(m, 18, mu=mu)
+    om = 0.55 + 0.02j
+    say("reflected tiles", case, ty, kz, xo, ft, bft, mu)
+    A = O.to_scipy(O.create_A(spec_of(m), om, eps, muu, ft=ft))
+    op = pkg.MaxwellOperator(m, om, ft=pkg.HH if ft else pkg.EE, εarr=eps, μarr=muu)
+    op.set_tile("fp64", ty, kz, xo)
+    rng = np.random.default_rng(11)
+    n = 3 * m.grid.M
+    X = rng.normal(size=(2, n)) + 1j * rng.normal(size=(2, n))
+    Y = op.apply(X)
+    op.close()
+    for r in range(2):
+        ref = A @ X[r]
+        assert np.linalg.norm(Y[r] - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("case,bft", [("rev_pml", (0, 0, 0)), ("rev_bloch", (0, 0, 0)), ("rev_slab", (0, 0, 0)),
+                                      ("rev_pml", (0, 1, 1)), ("rev_mixed", (1, 0, 0))])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_reflected(case, bft, precision, pkg, O):
+    """The HH formulation's solve on the reflected stencil (every axis, or
+    the axes whose boundft is EE, flipped) against the direct solve, and
+    against the same solve on the direct kernel."""
+    m = make_rev(pkg, case, bft)
+    muu, eps = rand_params(m, 23, mu=True)
+    om = 0.5 + 0.03j
+    A = O.to_scipy(O.create_A(spec_of(m), om, eps, muu, ft=1))
+    b = np.random.default_rng(12).normal(size=A.shape[0]) + 0j
+    say("solve reflected", case, bft, precision)
+    xd = sla.spsolve(A.tocsc(), b)
+    out = []
+    for ty in (4, 0):
+        op = pkg.MaxwellOperator(m, om, ft=pkg.HH, εarr=eps, μarr=muu)
+        op.set_tile("fp64", ty, 4, 1)
+        op.set_tile("fp32", ty, 4, 1)
+        h, st = op.solve(b, tol=1e-9, precision=precision)
+        op.close()
+        assert st.converged and np.linalg.norm(b - A @ h) / np.linalg.norm(b) <= 1.05e-9
+        assert np.linalg.norm(h - xd) / np.linalg.norm(xd) <= 1e-6
+        out.append(h)
+    assert np.linalg.norm(out[0] - out[1]) / np.linalg.norm(out[1]) <= 1e-6
+
+
 @pytest.mark.parametrize("case", OPCASES)
 @pytest.mark.parametrize("full,mu", [(False, False), (True, False), (False, True), (True, True)])
 def test_apply_matches_csc(case, full, mu, pkg, O):
