@@ -512,7 +512,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
     if (!skip && k0 < k1) {
         // a one-cell z axis (an extruded 2D model): its single plane is staged
         // once; its neighbours are that plane times the Bloch phase, or zero
-        const bool slab = PIPE && g.N[2] == 1;
+        const bool slab = (PIPE || OFFD) && g.N[2] == 1;
         const c64 pzn = g.bloch[2] ? g.ph[2] : czero<double>();
         // the owner's factors of H_x, H_y at plane k0−1 (loaded ahead of the
         // prefetch, as every per-thread load in the pipelined loop is)
@@ -530,8 +530,10 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
             }
         }
         if (slab) {   // one plane, staged once into buffer 1
-            fetch(0);
-            commit(1, 0, xc, qc, ec);
+            if (PIPE) {
+                fetch(0);
+                commit(1, 0, xc, qc, ec);
+            } else stage(1, 0, xc, qc, ec);
         } else if (PIPE) {
             fetch(k0 - 1);
             commit(0, k0 - 1, xn, qn, en);
@@ -562,9 +564,12 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
         }
         if (GD) pplane(1, 0, k0, k0 & 1);
         if (OFFD && own) {   // X_z of plane k0−1 at (i, j), (i+1, j), (i, j+1)
-            zkm0 = Xs[0][2][ty + 1][tx + 1];
-            zkmx = Xs[0][2][ty + 1][tx + 2];
-            zkmy = Xs[0][2][ty + 2][tx + 1];
+            // (slab: plane 0 times e^{+ikL}, or zero past a non-Bloch end)
+            const c64 pm = slab ? (g.bloch[2] ? g.phinv[2] : czero<double>()) : cone<double>();
+            const int b0 = slab ? 1 : 0;
+            zkm0 = pm * Xs[b0][2][ty + 1][tx + 1];
+            zkmx = pm * Xs[b0][2][ty + 1][tx + 2];
+            zkmy = pm * Xs[b0][2][ty + 2][tx + 1];
         }
         for (int k = k0; k < k1; ++k) {
             const int bk = (k - k0 + 1) & 1, bn = bk ^ 1;
@@ -657,9 +662,17 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
                     {
                         c64 s = (h5 * od.epso[3 * p + 1]) * (wx0 * Xs[bk][0][hj][hi] + wx1 * Xs[bk][0][hj][hi - 1]) +
                                 (h5 * od.epso[3 * p + 2]) * (wy0 * Xs[bk][1][hj][hi] + wy1 * Xs[bk][1][hj - 1][hi]);
-                        if (cpz >= 0)
-                            s = s + (h5 * od.epso[3 * cpz + 1]) * (wx0 * Xs[bn][0][hj][hi] + wx1 * Xs[bn][0][hj][hi - 1]) +
-                                (h5 * od.epso[3 * cpz + 2]) * (wy0 * Xs[bn][1][hj][hi] + wy1 * Xs[bn][1][hj - 1][hi]);
+                        if (cpz >= 0) {   // plane k+1 (slab: plane k times the phase)
+                            c64 ax_, ay_;
+                            if (slab) {
+                                ax_ = pzn * (wx0 * Xs[bk][0][hj][hi] + wx1 * Xs[bk][0][hj][hi - 1]);
+                                ay_ = pzn * (wy0 * Xs[bk][1][hj][hi] + wy1 * Xs[bk][1][hj - 1][hi]);
+                            } else {
+                                ax_ = wx0 * Xs[bn][0][hj][hi] + wx1 * Xs[bn][0][hj][hi - 1];
+                                ay_ = wy0 * Xs[bn][1][hj][hi] + wy1 * Xs[bn][1][hj - 1][hi];
+                            }
+                            s = s + (h5 * od.epso[3 * cpz + 1]) * ax_ + (h5 * od.epso[3 * cpz + 2]) * ay_;
+                        }
                         term[2] = s;
                     }
                     // X_z of this plane, for the next one

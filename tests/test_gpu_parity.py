@@ -45,6 +45,8 @@ CASES = {
                        kb=(0.0, 0.0, 0.35)),
     "slab_2d": dict(N=(66, 5, 1), npml=((0, 1, 0), (0, 1, 0)), om=0.5, bloch=(True, False, True),
                     kb=(0.1, 0.0, 0.0)),
+    "slab_pec": dict(N=(8, 6, 1), npml=((1, 0, 0), (1, 0, 0)), om=0.6, bloch=(False, True, False),
+                     kb=(0.0, 0.25, 0.0), nu=7),
 }
 
 
@@ -228,7 +230,7 @@ OPCASES = ["uniform_pml", "nonuniform_bloch", "mixed_pml_bloch", "tiny_bloch_axe
 
 
 @pytest.mark.parametrize("case", ["wide", "wide_bloch", "nonuniform_bloch", "tiny_bloch_axes", "mixed_pml_bloch",
-                                  "slab_bloch", "slab_2d"])
+                                  "slab_bloch", "slab_2d", "slab_pec"])
 @pytest.mark.parametrize("ty,kz,xo", [(1, 1, 1), (1, 4, 0), (2, 3, 1), (2, 16, 0), (4, 2, 1), (4, 16, 0),
                                       (4, 16, 1), (8, 3, 1), (8, 16, 0), (0, 1, 0),
                                       (1, 3, 3), (2, 5, 2), (4, 16, 3), (8, 2, 3)])
