@@ -636,8 +636,9 @@ def run_mixed_fgmres(a):
 def run_next_rows(a):
     """The rows beyond the plain EE path, on the 64³ random-box model (FP64 to
     1e-8): the EE solve (tiled stencil) next to the HH formulation and a
-    boundft = HH z axis (two-pass direct kernel), h_from_e / e_from_h,
-    create_Mcs, and a 1024² 2D TE solve (the single-z-cell 3D path)."""
+    boundft = HH z axis (the tiled stencil on the reflected grid), h_from_e /
+    e_from_h, create_Mcs, and a 1024² 2D TE solve (the single-z-cell
+    operator restricted to Ex, Ey), with the default and the autotuned tile."""
     import numpy as np
     import maxwellfdfd as M
     from maxwellfdfd import twod as T
@@ -704,9 +705,14 @@ def run_next_rows(a):
     T.calc_matparams(m2d)
     je2d, jm2d = T.create_srcs(m2d)
     op2d = T.MaxwellOperator2D(m2d, om)
-    _, out["te2d_1024_solve"] = solve(op2d, T.create_b(M.EE, om, m2d, je2d, jm2d))
-    op2d.close()
+    b2d = T.create_b(M.EE, om, m2d, je2d, jm2d)
+    _, out["te2d_1024_solve"] = solve(op2d, b2d)
     log("2D TE 1024^2", out["te2d_1024_solve"])
+    t64, _, _ = op2d.op.autotune(1, 10)
+    out["te2d_1024_tile_autotuned"] = list(t64)
+    _, out["te2d_1024_solve_autotuned"] = solve(op2d, b2d)
+    op2d.close()
+    log("2D TE 1024^2 autotuned", t64, out["te2d_1024_solve_autotuned"])
     return out
 
 

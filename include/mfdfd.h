@@ -219,6 +219,18 @@ int mfd_op_apply_dev(mfd_op *op, const void *x, void *y, int64_t nrhs, int prec)
  * default (opt-in; eligible with diagonal ε, μ = 1, ω ≠ 0).  No reference
  * counterpart: the reference has no solver (model.jl stops at create_A). */
 int mfd_op_set_graddiv(mfd_op *op, int enable);
+/* Unknowns of the 2D and 1D models (replaces the 3D-embedded solve of
+ * ModelTE te.jl:4-13, E = (Ex, Ey): c0 = 0, nc = 2; ModelTM tm.jl:4-13,
+ * E = Ez: c0 = 2, nc = 1; ModelTEM tem.jl:4-13, E = Ex: c0 = 0, nc = 1):
+ * restrict the operator of their single-cell 3D model to the E components
+ * c0 .. c0+nc−1, i.e. to the model's own create_A, A[w,w].  Afterwards every
+ * apply and solve takes vectors of nc·M values in the component-last order
+ * (p + M·k for component c0 + k), and the solver moves nc/3 of the bytes.
+ * Needs an operator created with order_cmpfirst = 0, μ = 1 and boundft = EE
+ * on every axis (MFD_ERR_UNSUPPORTED otherwise), grad-div off; h_from_e,
+ * e_from_h, the adjoint and TY = 0 then return MFD_ERR_UNSUPPORTED.
+ * c0 = 0, nc = 3 restores the full vectors. */
+int mfd_op_set_components(mfd_op *op, int c0, int nc);
 /* Tile-shape autotuner for the stencil (rows per block TY, z-planes per block
  * KZ, tile order: bit 0 set = each XCD gets a contiguous run of tiles, clear =
  * dispatch order; bit 1 set = 32-wide tiles of 2·TY rows, a candidate when

@@ -11,6 +11,7 @@ int op_destroy_impl(mfd_op *op);
 int op_apply_impl(mfd_op *op, const mfd_c64 *x, mfd_c64 *y, int64_t nrhs);
 int op_apply_dev_impl(mfd_op *op, const void *x, void *y, int64_t nrhs, int prec);
 int op_set_graddiv_impl(mfd_op *op, int enable);
+int op_set_components_impl(mfd_op *op, int c0, int nc);
 int op_set_tile_impl(mfd_op *op, int prec, int ty, int kz, int xcd);
 int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best32, double *ms_out);
 int op_bench_apply_impl(mfd_op *op, int64_t nrhs, int prec, int iters, double *ms);
@@ -342,6 +343,11 @@ int mfd_op_set_graddiv(mfd_op *op, int enable) {
     if (!op) { set_error("null operator"); return MFD_ERR_ARG; }
     CHECK_CTX(op->ctx);
     return op_set_graddiv_impl(op, enable);
+}
+int mfd_op_set_components(mfd_op *op, int c0, int nc) {
+    if (!op) { set_error("null operator"); return MFD_ERR_ARG; }
+    CHECK_CTX(op->ctx);
+    return op_set_components_impl(op, c0, nc);
 }
 int mfd_op_set_tile(mfd_op *op, int prec, int ty, int kz, int xcd_order) {
     if (!op) { set_error("null operator"); return MFD_ERR_ARG; }

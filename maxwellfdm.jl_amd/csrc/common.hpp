@@ -85,6 +85,9 @@ struct DGrid {
     // axes along which this view is the reflection of the operator's grid
     // (bit a: index i of axis a is the caller's N_a−1−i); 0 for the grid itself
     int rmask = 0;
+    // component window of the DOF vectors: components c0 .. c0+nc−1 only
+    // (nc < 3 with the component-last order: the 2D and 1D models' unknowns)
+    int c0 = 0, nc = 3;
 };
 
 __host__ __device__ inline int g0type(const int *bft, int ft, int a) {

@@ -36,10 +36,11 @@ struct DevGrid {
 template <> inline const DGrid<double> &DevGrid::view<double>() const { return d64; }
 template <> inline const DGrid<float> &DevGrid::view<float>() const { return d32; }
 
-// DOF index of component c at cell p
+// DOF index of component c at cell p (component-last vectors start at the
+// window's first component)
 template <typename R>
 __host__ __device__ inline int64_t dofidx(const DGrid<R> &g, int c, int64_t p) {
-    return g.cmpfirst ? (int64_t)c + 3 * p : p + g.M * c;
+    return g.cmpfirst ? (int64_t)c + 3 * p : p + g.M * (c - g.c0);
 }
 
 }  // namespace mfd

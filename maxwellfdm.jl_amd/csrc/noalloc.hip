@@ -57,7 +57,7 @@ using namespace mfd;
 int solve_dev_noalloc_impl(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_solve_opts *oin,
                            mfd_solve_stats *st) {
     if (nrhs < 1 || nrhs > 65535) { set_error("nrhs out of range"); return MFD_ERR_ARG; }
-    const int64_t n = 3 * op->G.h.M;
+    const int64_t n = op->n;
     const size_t need = ws_bytes(n, nrhs);
     if (need > NOALLOC_SCRATCH_BYTES) {
         char msg[160];

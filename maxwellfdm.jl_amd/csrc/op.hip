@@ -45,7 +45,7 @@ __device__ inline void cell3(const DGrid<R> &g, int64_t p, int &i, int &j, int &
 template <typename R>
 __device__ inline void dof2cp(const DGrid<R> &g, int64_t e, int &c, int64_t &p) {
     if (g.cmpfirst) { c = (int)(e % 3); p = e / 3; }
-    else { c = (int)(e / g.M); p = e - (int64_t)c * g.M; }
+    else { const int k = (int)(e / g.M); c = g.c0 + k; p = e - (int64_t)k * g.M; }
 }
 // T (or T⁻¹) for E_c at (i,j,k): product over axes of sqrt(s∆l) at E_c's grid
 // type (which each axis's boundft sets)
@@ -1076,7 +1076,7 @@ void ws_carve(Workspace &w, void *base, int64_t n, int64_t nrhs) {
 
 static int ws_ensure(mfd_op *op, int64_t nrhs, int prec) {
     Workspace &w = op->ws;
-    int64_t n = 3 * op->G.h.M;
+    int64_t n = op->n;
     if (w.fixed) {   // a caller-provided workspace: never reallocated here
         if (w.nrhs != nrhs || w.n != n) { set_error("fixed workspace does not match nrhs"); return MFD_ERR_ARG; }
         w.prec = prec;
@@ -1114,7 +1114,7 @@ static int launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int6
                         bool finish = true) {
     const DGrid<R> &g = op->G.view<R>();
     const OpData<R> &od = opd<R>(op);
-    int64_t vs = 3 * g.M;
+    int64_t vs = op->n;
     hipStream_t s = op->stream;
     int ty = std::is_same<R, double>::value ? op->tile64[0] : op->tile32[0];
     int kz = std::is_same<R, double>::value ? op->tile64[1] : op->tile32[1];
@@ -1132,7 +1132,7 @@ static int launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int6
     if ((op->ee_all || op->rev) && ty > 0 && !(op->has_mu && op->full)) {
         // order bit 1: 32-wide tiles of 2·TY rows (same threads per block);
         // the off-diagonal, grad-div, μ and reflected stencils are built 64-wide only
-        const bool narrow = (xr & 2) && !op->full && !(MODE == MODE_P && op->gd) && !op->has_mu && !op->rev;
+        const bool narrow = (xr & 2) && !op->full && !(MODE == MODE_P && op->gd) && !op->has_mu && !op->rev && !op->win;
         xr &= 1;
         const int tw = narrow ? 32 : 64, rows = narrow ? 2 * ty : ty;
         int nx = (g.N[0] + tw - 1) / tw, ny = (g.N[1] + rows - 1) / rows;
@@ -1140,7 +1140,17 @@ static int launch_apply(mfd_op *op, const cx<R> *x, cx<R> *y, const c64 *b, int6
         while ((int64_t)nx * ny * nch > NPART) { kz *= 2; nch = (g.N[2] + kz - 1) / kz; }
         dim3 grid(nx, ny, (unsigned)(nch * nrhs));
         const DGrid<double> &gr = op->G.r64;
-        if (op->rev && op->has_mu) {   // boundft HH on some axis: the grid reflected along those axes
+        if (op->win && op->full) {   // a component window (2D and 1D models), μ = 1, boundft EE
+            if (ty == 8) k_apply_tiled<R, MODE, EPI, 8, false, false, true, 64, false, false, true><<<grid, 512, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, false, false, true, 64, false, false, true><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, false, false, true, 64, false, false, true><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else k_apply_tiled<R, MODE, EPI, 1, false, false, true, 64, false, false, true><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+        } else if (op->win) {
+            if (ty == 8) k_apply_tiled<R, MODE, EPI, 8, false, false, false, 64, false, false, true><<<grid, 512, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, false, false, false, 64, false, false, true><<<grid, 128, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, false, false, false, 64, false, false, true><<<grid, 256, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+            else k_apply_tiled<R, MODE, EPI, 1, false, false, false, 64, false, false, true><<<grid, 64, 0, s>>>(op->G.d64, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
+        } else if (op->rev && op->has_mu) {   // boundft HH on some axis: the grid reflected along those axes
             if (ty == 8) k_apply_tiled<R, MODE, EPI, 8, false, false, false, 64, true, true><<<grid, 512, 0, s>>>(gr, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
             else if (ty == 2) k_apply_tiled<R, MODE, EPI, 2, false, false, false, 64, true, true><<<grid, 128, 0, s>>>(gr, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
             else if (ty == 4) k_apply_tiled<R, MODE, EPI, 4, false, false, false, 64, true, true><<<grid, 256, 0, s>>>(gr, op->d64, op->d64.om2, x, y, b, vs, kz, nch, part, scal, mask, xr);
@@ -1287,6 +1297,7 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
     if (rc) { delete op; return rc; }
     hipStream_t s = ctx->stream;
     int64_t M = op->G.h.M;
+    op->n = 3 * M;
     op->omega = mk<double>(omega.re, omega.im);
     c64 om2 = op->omega * op->omega;
     DevArray E, U;
@@ -1446,6 +1457,7 @@ int op_create_csc_impl(mfd_ctx *ctx, const mfd_csc *A, mfd_op **out) {
     int rc = op->G.create(&g, ctx->stream);
     if (rc) return bail(rc);
     hipStream_t s = ctx->stream;
+    op->n = n;
     op->csr = 1;
     op->csr_buf = R.buf;
     op->csr_rowptr = R.rowptr;
@@ -1524,6 +1536,38 @@ int op_create_csc_impl(mfd_ctx *ctx, const mfd_csc *A, mfd_op **out) {
     return MFD_OK;
 }
 
+// Component window (the 2D and 1D models: te.jl:4-13 E = (Ex, Ey), tm.jl:4-13
+// E = Ez, tem.jl:4-13 E = Ex): the unknowns are the E components c0 ..
+// c0+nc−1 of the single-cell 3D operator, whose restriction A[w,w] to them is
+// the model's create_A.  Vectors hold nc·M values, component-last (p + M·k
+// for component c0 + k).  The tiled stencil reads the window's components
+// (the others are zero) and writes only them; the per-DOF solver kernels
+// find each unknown's component through the window.
+int op_set_components_impl(mfd_op *op, int c0, int nc) {
+    if (op->csr) { set_error("components: an operator built from a CSC matrix has no component layout"); return MFD_ERR_UNSUPPORTED; }
+    if (c0 < 0 || nc < 1 || c0 + nc > 3) { set_error("components: need 0 <= c0, 1 <= nc, c0 + nc <= 3"); return MFD_ERR_ARG; }
+    if (op->G.h.cmpfirst && nc < 3) { set_error("components: a window needs the component-last order (order_cmpfirst = 0)"); return MFD_ERR_ARG; }
+    if ((op->has_mu || op->rev) && nc < 3) {
+        set_error("components: a window is built for μ = 1 and boundft = EE on every axis");
+        return MFD_ERR_UNSUPPORTED;
+    }
+    if (op->gd && nc < 3) { set_error("components: switch the grad-div form off first"); return MFD_ERR_UNSUPPORTED; }
+    MFD_HIP(hipStreamSynchronize(op->stream));
+    for (DGrid<double> *g : {&op->G.d64, &op->G.r64}) { g->c0 = c0; g->nc = nc; }
+    op->G.d32.c0 = c0;
+    op->G.d32.nc = nc;
+    op->n = (int64_t)nc * op->G.h.M;
+    op->win = nc < 3;
+    // the grad-div form's node scratch and transforms take all components
+    op->gd_ok = !op->win && !op->full && !op->has_mu && op->ee_all &&
+                !(op->d64.om2.re == 0.0 && op->d64.om2.im == 0.0);
+    if (op->win)
+        for (int *t : {op->tile64, op->tile32})
+            if (t[0] == 0) { t[0] = 1; t[1] = 4; t[2] = 1; }
+    op->ws.release();   // sized for the old vector length
+    return MFD_OK;
+}
+
 int op_set_graddiv_impl(mfd_op *op, int enable) {
     if (enable && !op->gd_ok) {
         set_error("grad-div form needs diagonal ε, μ = 1 and ω ≠ 0");
@@ -1555,7 +1599,7 @@ int op_apply_dev_impl(mfd_op *op, const void *x, void *y, int64_t nrhs, int prec
 }
 
 int op_apply_impl(mfd_op *op, const mfd_c64 *x, mfd_c64 *y, int64_t nrhs) {
-    int64_t n = 3 * op->G.h.M;
+    int64_t n = op->n;
     size_t b = sizeof(c64) * n * nrhs;
     void *dx = nullptr, *dy = nullptr;
     MFD_HIP(hipMalloc(&dx, b));
@@ -1574,7 +1618,7 @@ int op_apply_impl(mfd_op *op, const mfd_c64 *x, mfd_c64 *y, int64_t nrhs) {
 // average ms of one stencil launch (EPI_STORE): MODE_A or MODE_P, FP64 or FP32 vectors
 static int time_apply(mfd_op *op, int64_t nrhs, int prec, int mode, int iters, double *ms) {
     MFD_TRY(hbuf_ensure(op, nrhs));
-    int64_t n = 3 * op->G.h.M;
+    int64_t n = op->n;
     size_t es = prec == MFD_FP64 ? sizeof(c64) : sizeof(c32);
     void *dx = nullptr, *dy = nullptr;
     MFD_HIP(hipMalloc(&dx, es * n * nrhs));
@@ -1689,7 +1733,7 @@ int op_autotune_impl(mfd_op *op, int64_t nrhs, int iters, int *best64, int *best
     // size that is a multiple of 64 they only add halo columns: at 128³ they
     // tie on the Krylov step within noise and lose 19% on the A apply)
     const int tys[4] = {1, 2, 4, 8}, kzs[4] = {4, 8, 16, 32};
-    const int norders = (op->G.h.N[0] % 64 != 0 && !op->full && !op->gd && !op->has_mu && !op->rev) ? 4 : 2;
+    const int norders = (op->G.h.N[0] % 64 != 0 && !op->full && !op->gd && !op->has_mu && !op->rev && !op->win) ? 4 : 2;
     double bestms[2] = {1e30, 1e30};
     int bt[2][3] = {{4, 16, 1}, {4, 16, 1}};
     const int its = std::max(2, std::min(iters, 50));
@@ -1893,14 +1937,14 @@ static void launch_cocg_iter(mfd_op *op, int64_t nrhs, int par) {
     cx<R> *po = cocg_p<R>(w, par), *pn = cocg_p<R>(w, par ^ 1);
     const DGrid<R> &g = op->G.view<R>();
     int ty = std::is_same<R, double>::value ? op->tile64[0] : op->tile32[0];
-    if (!op->csr && !op->full && !op->has_mu && !op->gd && op->ee_all && ty > 0) {
+    if (!op->csr && !op->full && !op->has_mu && !op->gd && op->ee_all && !op->win && ty > 0) {
         int kz = std::is_same<R, double>::value ? op->tile64[1] : op->tile32[1];
         // the fused COCG stencil is built 64-wide only: the narrow-tile bit is ignored
         int xr = (std::is_same<R, double>::value ? op->tile64[2] : op->tile32[2]) & 1;
         int nx = (g.N[0] + 63) / 64, ny = (g.N[1] + ty - 1) / ty, nch = (g.N[2] + kz - 1) / kz;
         while ((int64_t)nx * ny * nch > NPART) { kz *= 2; nch = (g.N[2] + kz - 1) / kz; }
         dim3 grid(nx, ny, (unsigned)(nch * nrhs));
-        int64_t vs = 3 * g.M;
+        int64_t vs = op->n;
         const cx<R> *rr = (const cx<R> *)w.r;
         cx<R> *xi = (cx<R> *)w.z, *t = (cx<R> *)w.t;
         if (ty == 8)
@@ -2101,6 +2145,7 @@ static int run_cocg(mfd_op *op, const void *b, void *e, int64_t nrhs, const mfd_
 
 int op_set_tile_impl(mfd_op *op, int prec, int ty, int kz, int xcd) {
     if (op->csr) { set_error("tile: an operator built from a CSC matrix has no stencil tiles"); return MFD_ERR_UNSUPPORTED; }
+    if (op->win && ty == 0) { set_error("tile: an operator with a component window runs the tiled stencil only (TY > 0)"); return MFD_ERR_UNSUPPORTED; }
     if (!(ty == 0 || ty == 1 || ty == 2 || ty == 4 || ty == 8) || kz < 1 || xcd < 0 || xcd > 3) {
         set_error("tile: TY in {0,1,2,4,8}, KZ >= 1, order in {0,1,2,3} (bit 0: XCD order, bit 1: 32-wide tiles)");
         return MFD_ERR_ARG;
@@ -3005,7 +3050,7 @@ int solve_batch_impl(int count, mfd_op *const *ops, const mfd_c64 *const *b, mfd
     std::vector<void *> db(count, nullptr), de(count, nullptr);
     int rc = MFD_OK;
     for (int p = 0; p < count && !rc; ++p) {
-        size_t bytes = sizeof(c64) * 3 * ops[p]->G.h.M;
+        size_t bytes = sizeof(c64) * ops[p]->n;
         if (hipMalloc(&db[p], bytes) != hipSuccess || hipMalloc(&de[p], bytes) != hipSuccess) { set_error("hipMalloc failed"); rc = MFD_ERR_ALLOC; break; }
         if (hipMemcpyAsync(db[p], b[p], bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
             hipMemcpyAsync(de[p], e[p], bytes, hipMemcpyHostToDevice, s) != hipSuccess) { set_error("copy to device failed"); rc = MFD_ERR_HIP; }
@@ -3013,7 +3058,7 @@ int solve_batch_impl(int count, mfd_op *const *ops, const mfd_c64 *const *b, mfd
     if (!rc) rc = solve_batch_dev_impl(count, ops, db.data(), de.data(), o, st);
     if (rc == MFD_OK || rc == MFD_ERR_NOCONV) {
         for (int p = 0; p < count; ++p) {
-            size_t bytes = sizeof(c64) * 3 * ops[p]->G.h.M;
+            size_t bytes = sizeof(c64) * ops[p]->n;
             if (hipMemcpyAsync(e[p], de[p], bytes, hipMemcpyDeviceToHost, s) != hipSuccess) rc = MFD_ERR_HIP;
         }
         if (hipStreamSynchronize(s) != hipSuccess) rc = MFD_ERR_HIP;
@@ -3026,7 +3071,7 @@ int solve_batch_impl(int count, mfd_op *const *ops, const mfd_c64 *const *b, mfd
 }
 
 int solve_impl(mfd_op *op, const mfd_c64 *b, mfd_c64 *e, int64_t nrhs, const mfd_solve_opts *o, mfd_solve_stats *st) {
-    int64_t n = 3 * op->G.h.M;
+    int64_t n = op->n;
     size_t bytes = sizeof(c64) * n * nrhs;
     void *db = nullptr, *de = nullptr;
     MFD_HIP(hipMalloc(&db, bytes));
@@ -3047,6 +3092,7 @@ int solve_impl(mfd_op *op, const mfd_c64 *b, mfd_c64 *e, int64_t nrhs, const mfd
 }
 
 int solve_adjoint_impl(mfd_op *op, const mfd_c64 *gvec, mfd_c64 *lam, const mfd_solve_opts *o, mfd_solve_stats *st) {
+    if (op->win) { set_error("adjoint solve: an operator with a component window (2D/1D model) is not supported"); return MFD_ERR_UNSUPPORTED; }
     if (!op->sym) {
         set_error("adjoint solve: the operator is not complex-symmetric (complex Bloch phase), so Aᵀ is not S A S⁻¹");
         return MFD_ERR_UNSUPPORTED;
@@ -3157,6 +3203,7 @@ int field_convert_impl(mfd_op *op, int which, const mfd_c64 *in, const mfd_c64 *
         set_error("h_from_e/e_from_h: operator of the EE formulation");
         return MFD_ERR_UNSUPPORTED;
     }
+    if (op->win) { set_error("h_from_e/e_from_h: the field conversions take all three components (no component window)"); return MFD_ERR_UNSUPPORTED; }
     if (which == 1 && op->full) { set_error("e_from_h: only diagonal ε is supported (Pε \\ is a sparse solve otherwise)"); return MFD_ERR_UNSUPPORTED; }
     if (op->omega.re == 0.0 && op->omega.im == 0.0) { set_error("h_from_e/e_from_h need ω ≠ 0"); return MFD_ERR_ARG; }
     int64_t n = 3 * op->G.h.M;
@@ -3191,6 +3238,7 @@ int field_convert_impl(mfd_op *op, int which, const mfd_c64 *in, const mfd_c64 *
 
 int adjoint_grad_impl(mfd_op *op, const mfd_c64 *e, const mfd_c64 *lam, double *grad) {
     if (op->csr) { set_error("adjoint_grad: needs an operator built from a model (ω and the grid)"); return MFD_ERR_UNSUPPORTED; }
+    if (op->win) { set_error("adjoint_grad: an operator with a component window (2D/1D model) is not supported"); return MFD_ERR_UNSUPPORTED; }
     int64_t n = 3 * op->G.h.M;
     void *d = nullptr;
     MFD_HIP(hipMalloc(&d, (2 * sizeof(c64) + sizeof(double)) * n));

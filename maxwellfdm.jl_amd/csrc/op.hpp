@@ -88,6 +88,8 @@ struct mfd_op {
     size_t hbuf_bytes = 0;
     int ee_all = 1;           // boundft = EE on every axis (the tiled stencils and field conversions need it)
     int rev = 0;              // axes with boundft = HH (bit a): the tiled stencil runs on G.r64, reflected along them
+    int64_t n = 0;            // unknowns per right-hand side (3M, or nc·M with a component window)
+    int win = 0;              // a component window is set (mfd_op_set_components)
     // an operator built from a caller's CSC matrix A (mfd_op_create_csc):
     // row-major copy of A, applied by a sparse product; its grid is a 1D unit
     // grid of n/3 cells that only provides the DOF numbering (T = 1)

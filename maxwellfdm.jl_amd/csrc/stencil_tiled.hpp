@@ -138,8 +138,10 @@ __device__ inline void plane_init(const DGrid<R> &g, int vk, PlaneZ<R> &z) {
 // caller's cell (index N_a−1−i along a reflected axis), so the vectors keep
 // their order, and the E components along reflected axes are negated on
 // load and on store.
+// WIN: the vectors hold only the components c0 .. c0+nc−1 of g (the 2D and 1D
+// models); the others read as zero and are not written.
 template <typename R, int MODE, int EPI, int TY, bool GD, bool FUSE = false, bool OFFD = false, int TXW = 64,
-          bool MU = false, bool REV = false>
+          bool MU = false, bool REV = false, bool WIN = false>
 __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<double> g, OpData<double> od, c64 om2,
                                                          const cx<R> *__restrict__ xall,
                                                          cx<R> *__restrict__ yall, const c64 *__restrict__ ball,
@@ -150,6 +152,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
                                                          cx<R> *__restrict__ itall = nullptr) {
     using C = TileCfg<TY, TXW>;
     static_assert(!(REV && (GD || OFFD || FUSE)), "the reflected form serves diagonal ε, unfused, without grad-div");
+    static_assert(!(WIN && (GD || FUSE || MU || REV)), "the windowed form serves μ = 1, boundft EE, unfused");
     constexpr int GB = GD ? 2 : 1, GH = GD ? C::XH : 1, GW = GD ? C::XW : 1;
     __shared__ c64 Xs[2][3][C::XH][C::XW];
     __shared__ c64 Ys[GB][3][GH][GW];
@@ -217,6 +220,8 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
     auto gz = [&](int ck) -> int64_t { return NxNy * (REV && (g.rmask & 4) ? g.N[2] - 1 - ck : ck); };
     // REV: E_c is negated along a reflected axis c
     auto rsg = [&](int c, c64 v) -> c64 { return REV && ((g.rmask >> c) & 1) ? -v : v; };
+    // component window of the vectors (the others read as zero, are not written)
+    auto inw = [&](int c) -> bool { return !WIN || (unsigned)(c - g.c0) < (unsigned)g.nc; };
 
     // staged points of this thread: its interior point (slot 0) and its share
     // of the halo ring (slots 1..NRS; the ring has 2·XW + 2·TY points)
@@ -362,7 +367,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
                             pn[dof] = cvt<R>(xv);
                             itv[dof] = cvt<R>(cvt<double>(itv[dof]) + fapend * po);
                         }
-                    } else xv = rsg(c, cvt<double>(x[dof]));
+                    } else xv = inw(c) ? rsg(c, cvt<double>(x[dof])) : czero<double>();
                     c64 qv = cone<double>();
                     if (MODE == MODE_P) {
                         qv = od.qs[3 * p + c];   // q·T⁻¹
@@ -401,7 +406,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
             const int64_t p = gxy(sp[s].gi, sp[s].gj) + gz(z.gk);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                pfx[s][c] = ok ? x[dofidx(g, c, p)] : mk<R>((R)0, (R)0);
+                pfx[s][c] = ok && inw(c) ? x[dofidx(g, c, p)] : mk<R>((R)0, (R)0);
                 if (MODE == MODE_P) pfq[s][c] = ok ? od.qs[3 * p + c] : czero<double>();
                 if ((MODE == MODE_A || MU) && s == 0) pfe[c] = ok ? od.epsd[3 * p + c] : cone<double>();
             }
@@ -703,6 +708,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
                         out = w * in;
                         if (!GD && !MU) out = out + xc[u];
                     }
+                    if (!inw(u)) continue;
                     int64_t e = dofidx(g, u, REV ? gxy(i, j) + gz(k) : p);
                     if (EPI == EPI_RESID) {
                         const c64 oc = rsg(u, out);   // in the caller's frame

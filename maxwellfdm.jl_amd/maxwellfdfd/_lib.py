@@ -105,6 +105,7 @@ SIGNATURES = {
     "mfd_op_apply": (C.c_int, [_P, _P, _P, C.c_int64]),
     "mfd_op_apply_dev": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int]),
     "mfd_op_set_graddiv": (C.c_int, [_P, C.c_int]),
+    "mfd_op_set_components": (C.c_int, [_P, C.c_int, C.c_int]),
     "mfd_op_set_tile": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "mfd_op_autotune": (C.c_int, [_P, C.c_int64, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                   C.POINTER(C.c_double)]),

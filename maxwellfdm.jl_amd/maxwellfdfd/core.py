@@ -599,13 +599,15 @@ class MaxwellOperator:
     iterations with FP64 residual and update).
     """
 
-    def __init__(self, mdl, ω, ft=EE, εarr=None, μarr=None):
+    def __init__(self, mdl, ω, ft=EE, εarr=None, μarr=None, order_cmpfirst=None):
         self.mdl = mdl
         self.ω = complex(ω)
         self.n = 3 * mdl.grid.M
         e = L.carr(mdl.εarr if εarr is None else εarr, "F")
         u = L.carr(mdl.μarr if μarr is None else μarr, "F")
         g = mdl._desc()
+        if order_cmpfirst is not None:   # DOF order of this operator's vectors, else the model's
+            g.order_cmpfirst = int(bool(order_cmpfirst))
         h = C.c_void_p()
         check(lib().mfd_op_create(context(), C.byref(g), ft, L.cplx(ω), ptr(e), ptr(u), C.byref(h)))
         self._h = h
@@ -655,6 +657,12 @@ class MaxwellOperator:
     def apply_dev(self, x_ptr, y_ptr, nrhs=1, precision="fp64"):
         check(lib().mfd_op_apply_dev(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), nrhs,
                                      L.MFD_FP64 if precision == "fp64" else L.MFD_MIXED))
+
+    def set_components(self, c0, nc):
+        """Restrict the unknowns to the E components c0 .. c0+nc−1 (the 2D and 1D
+        models; see mfd_op_set_components): vectors of nc·M values, component-last."""
+        check(lib().mfd_op_set_components(self._h, int(c0), int(nc)))
+        self.n = int(nc) * self.mdl.grid.M
 
     def set_graddiv(self, enable=True):
         """Grad-div augmented inner operator of the solver (see mfd_op_set_graddiv)."""

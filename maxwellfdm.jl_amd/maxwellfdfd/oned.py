@@ -17,6 +17,7 @@ import numpy as np
 from . import core as M
 from ._lib import MfdArgumentError
 from . import _lib as L
+from .twod import _WindowOperator
 
 ECMP = (0,)   # Ex
 HCMP = (1,)   # Hy
@@ -166,36 +167,8 @@ def create_linsys(ft, ω, mdl):
     return create_A(ft, ω, mdl), create_b(ft, ω, mdl, je, jm)
 
 
-class MaxwellOperator1D:
-    """Device operator and solver of the 1D model (the 3D single-column
-    operator with the Ey and Ez right-hand sides zero)."""
+class MaxwellOperator1D(_WindowOperator):
+    """Device operator and solver of the 1D model: the Ex unknowns only."""
 
     def __init__(self, mdl, ω):
-        self.mdl = mdl
-        self.op = M.MaxwellOperator(mdl.m3, ω)
-        self.sel = mdl.esel()
-        self.n = len(self.sel)
-
-    def close(self):
-        self.op.close()
-
-    def _embed(self, x):
-        x = np.atleast_2d(np.asarray(x, dtype=np.complex128))
-        X = np.zeros((x.shape[0], self.op.n), dtype=np.complex128)
-        X[:, self.sel] = x
-        return X
-
-    def apply(self, x):
-        y = self.op.apply(self._embed(x))
-        return np.atleast_2d(y)[:, self.sel].reshape(np.shape(x))
-
-    def solve(self, b, **kw):
-        e3, st = self.op.solve(self._embed(b), **kw)
-        return np.atleast_2d(e3)[:, self.sel].reshape(np.shape(b)), st
-
-    def h_from_e(self, e, jm=None):
-        jm3 = None
-        if jm is not None:
-            jm3 = np.zeros(self.op.n, dtype=np.complex128)
-            jm3[self.mdl.hsel()] = jm
-        return self.op.h_from_e(self._embed(e)[0], jm3)[self.mdl.hsel()]
+        super().__init__(mdl, ω, mdl.m3, ECMP, mdl.esel(), mdl.hsel())

@@ -15,7 +15,7 @@ from typing import Tuple
 import numpy as np
 
 from . import core as M
-from ._lib import MfdArgumentError
+from ._lib import MfdArgumentError, MfdUnsupportedError
 from . import _lib as L
 
 
@@ -197,36 +197,74 @@ def create_linsys(ft, ω, mdl):
     return create_A(ft, ω, mdl), create_b(ft, ω, mdl, je, jm)
 
 
-class MaxwellOperator2D:
-    """Device operator and solver of a 2D model (the 3D single-cell operator
-    with the other polarisation's right-hand side zero)."""
+class _WindowOperator:
+    """Device operator and solver of a 2D or 1D model: the single-cell 3D
+    operator restricted to the model's E components (mfd_op_set_components),
+    so the vectors hold K values per cell, not 3.  Model vectors are
+    component-first (c + K·p); the device vectors are component-last
+    (p + M·k), and the two orders are exchanged on the host.  Where the
+    window is not built (μ ≠ 1, boundft = HH) the full single-cell operator
+    serves, with the model's components embedded in three per cell."""
 
-    def __init__(self, mdl, ω):
-        self.mdl = mdl
-        self.op = M.MaxwellOperator(mdl.m3, ω)
-        self.sel = mdl.esel()
-        self.n = len(self.sel)
+    def __init__(self, mdl, ω, m3, ecmp, esel, hsel):
+        self.mdl, self.ω, self._m3 = mdl, ω, m3
+        self.K = len(ecmp)
+        self.Mc = m3.grid.M
+        self.n = self.K * self.Mc
+        self._esel, self._hsel = esel, hsel
+        self._full = None   # the 3D operator, for the field conversions
+        self.op = M.MaxwellOperator(m3, ω, order_cmpfirst=False)
+        try:
+            self.op.set_components(ecmp[0], self.K)
+            self.windowed = True
+        except MfdUnsupportedError:
+            self.op.close()
+            self.op = self._full = M.MaxwellOperator(m3, ω)
+            self.windowed = False
 
     def close(self):
         self.op.close()
+        if self._full is not None and self._full is not self.op:
+            self._full.close()
 
-    def _embed(self, x):
+    def _to_dev(self, x):
         x = np.atleast_2d(np.asarray(x, dtype=np.complex128))
-        X = np.zeros((x.shape[0], self.op.n), dtype=np.complex128)
-        X[:, self.sel] = x
-        return X
+        if not self.windowed:
+            X = np.zeros((x.shape[0], 3 * self.Mc), dtype=np.complex128)
+            X[:, self._esel] = x
+            return X
+        return np.ascontiguousarray(x.reshape(-1, self.Mc, self.K).transpose(0, 2, 1)).reshape(-1, self.n)
+
+    def _from_dev(self, y, shape):
+        y = np.atleast_2d(y)
+        if not self.windowed:
+            return y[:, self._esel].reshape(shape)
+        return np.ascontiguousarray(y.reshape(-1, self.K, self.Mc).transpose(0, 2, 1)).reshape(shape)
 
     def apply(self, x):
-        y = self.op.apply(self._embed(x))
-        return np.atleast_2d(y)[:, self.sel].reshape(np.shape(x))
+        return self._from_dev(self.op.apply(self._to_dev(x)), np.shape(x))
 
     def solve(self, b, **kw):
-        e3, st = self.op.solve(self._embed(b), **kw)
-        return np.atleast_2d(e3)[:, self.sel].reshape(np.shape(b)), st
+        if kw.get("x0") is not None:
+            kw["x0"] = self._to_dev(kw["x0"])
+        e, st = self.op.solve(self._to_dev(b), **kw)
+        return self._from_dev(e, np.shape(b)), st
 
     def h_from_e(self, e, jm=None):
+        if self._full is None:
+            self._full = M.MaxwellOperator(self._m3, self.ω)
+        e3 = np.zeros(3 * self.Mc, dtype=np.complex128)
+        e3[self._esel] = np.ravel(e)
         jm3 = None
         if jm is not None:
-            jm3 = np.zeros(self.op.n, dtype=np.complex128)
-            jm3[self.mdl.hsel()] = jm
-        return self.op.h_from_e(self._embed(e)[0], jm3)[self.mdl.hsel()]
+            jm3 = np.zeros(3 * self.Mc, dtype=np.complex128)
+            jm3[self._hsel] = jm
+        return self._full.h_from_e(e3, jm3)[self._hsel]
+
+
+class MaxwellOperator2D(_WindowOperator):
+    """Device operator and solver of a 2D model: (Ex, Ey) unknowns for
+    ModelTE, Ez for ModelTM."""
+
+    def __init__(self, mdl, ω):
+        super().__init__(mdl, ω, mdl.m3, mdl.ECMP, mdl.esel(), mdl.hsel())

@@ -927,7 +927,7 @@ def test_adjoint_design_region(pkg, O):
         assert grad[dof] == pytest.approx(fd, rel=1e-5, abs=1e-9 * np.abs(grad).max())
 
 
-def model2d(pkg, kind, N=(14, 11), npml=2, bloch=(False, False), om=0.6):
+def model2d(pkg, kind, N=(14, 11), npml=2, bloch=(False, False), om=0.6, mu=1.0):
     T = pkg.twod
     lp = nonuniform(N, 21)
     m = (T.ModelTE if kind == "te" else T.ModelTM)(T.Grid2(lp, bloch))
@@ -936,7 +936,7 @@ def model2d(pkg, kind, N=(14, 11), npml=2, bloch=(False, False), om=0.6):
     c = [0.5 * (a[0] + a[-1]) for a in lp]
     T.add_obj(m, "bg", T.Rect(tuple(c), (lp[0][-1] - lp[0][0], lp[1][-1] - lp[1][0])), ε=1.0)
     T.add_obj(m, "rod", T.Disk((c[0] + 0.7, c[1] - 0.4), 2.6), ε=9.0 + 0.2j)
-    T.add_obj(m, "bar", T.Rect((c[0] - 2.1, c[1] + 1.3), (1.7, 3.1)), ε=4.0)
+    T.add_obj(m, "bar", T.Rect((c[0] - 2.1, c[1] + 1.3), (1.7, 3.1)), ε=4.0, μ=mu)
     T.calc_matparams(m)
     return m
 
@@ -970,6 +970,104 @@ def test_model2d_operator_and_solve(kind, pkg, O):
     assert np.linalg.norm(e - xd) / np.linalg.norm(xd) <= 1e-7
     assert np.linalg.norm(b - A2 @ e) / np.linalg.norm(b) <= 1.05e-10
     op.close()
+
+
+def _oracle_2d(pkg, O, m, kind, om):
+    """The oracle's extruded 3D operator restricted to the 2D model's E
+    components, in the model's component-first order."""
+    s3 = spec_of(m.m3)
+    shapes, et, mt = shapes_of(m.m3)
+    ea, ma = O.matparams(s3, shapes, et, mt)
+    A3 = O.to_scipy(O.create_A(s3, om, ea, ma)).tocsr()
+    te = m.esel()
+    return A3[te][:, te].tocsc()
+
+
+@pytest.mark.parametrize("kind", ["te", "tm"])
+@pytest.mark.parametrize("bloch", [(False, False), (True, False)])
+@pytest.mark.parametrize("ty,kz,xo", [(1, 1, 1), (2, 4, 0), (4, 16, 1), (8, 2, 1)])
+def test_model2d_window_apply(kind, bloch, ty, kz, xo, pkg, O):
+    """The 2D operator with the component window (K = 2 unknowns per cell
+    for TE, 1 for TM, component-last on the device) against the oracle's
+    extruded operator restricted to the 2D DOFs, for several tile shapes."""
+    T = pkg.twod
+    om = 0.6
+    m = model2d(pkg, kind, bloch=bloch, om=om)
+    A2 = _oracle_2d(pkg, O, m, kind, om)
+    op = T.MaxwellOperator2D(m, om)
+    assert op.n == A2.shape[0] == (2 if kind == "te" else 1) * m.grid.M
+    op.op.set_tile("fp64", ty, kz, xo)
+    say("model2d window apply", kind, bloch, ty, kz, xo)
+    rng = np.random.default_rng(41)
+    X = rng.normal(size=(2, op.n)) + 1j * rng.normal(size=(2, op.n))
+    Y = op.apply(X)
+    op.close()
+    for r in range(2):
+        ref = A2 @ X[r]
+        assert np.linalg.norm(Y[r] - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("kind", ["te", "tm"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_model2d_window_solve(kind, precision, pkg, O):
+    """Two right-hand sides through the windowed 2D solve: each to 1e-9 in
+    the oracle's residual, against the direct solve, and against the solve of
+    the full single-cell 3D operator (three components per cell)."""
+    T = pkg.twod
+    om = 0.6
+    m = model2d(pkg, kind, om=om)
+    A2 = _oracle_2d(pkg, O, m, kind, om)
+    rng = np.random.default_rng(43)
+    B = rng.normal(size=(2, A2.shape[0])) + 1j * rng.normal(size=(2, A2.shape[0]))
+    op = T.MaxwellOperator2D(m, om)
+    say("model2d window solve", kind, precision)
+    E, st = op.solve(B, tol=1e-9, precision=precision)
+    op.close()
+    full = pkg.MaxwellOperator(m.m3, om)
+    sel = m.esel()
+    B3 = np.zeros((2, full.n), dtype=complex)
+    B3[:, sel] = B
+    E3, _ = full.solve(B3, tol=1e-9, precision=precision)
+    full.close()
+    for r in range(2):
+        assert np.linalg.norm(B[r] - A2 @ E[r]) / np.linalg.norm(B[r]) <= 1.05e-9
+        xd = sla.spsolve(A2, B[r])
+        assert np.linalg.norm(E[r] - xd) / np.linalg.norm(xd) <= 1e-6
+        assert np.linalg.norm(E[r] - E3[r][sel]) / np.linalg.norm(xd) <= 1e-6
+
+
+def test_model2d_window_fallback(pkg, O):
+    """μ ≠ 1: the window is not built for it, so the 2D operator falls back
+    to the full single-cell operator (three components per cell) and still
+    matches the oracle."""
+    T = pkg.twod
+    om = 0.6
+    m = model2d(pkg, "te", om=om, mu=2.0)
+    A2 = _oracle_2d(pkg, O, m, "te", om)
+    op = T.MaxwellOperator2D(m, om)
+    assert not op.windowed and op.n == A2.shape[0]
+    X = np.random.default_rng(44).normal(size=op.n) + 0j
+    Y = op.apply(X)
+    op.close()
+    ref = A2 @ X
+    assert np.linalg.norm(Y - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_component_window_refusals(pkg):
+    """A window needs the component-last order; a windowed operator refuses
+    the direct kernel and the three-component field conversions."""
+    m = model2d(pkg, "te")
+    op = pkg.MaxwellOperator(m.m3, 0.6)   # component-first
+    with pytest.raises(pkg.MfdArgumentError):
+        op.set_components(0, 2)
+    op.close()
+    w = pkg.twod.MaxwellOperator2D(m, 0.6)
+    assert w.windowed
+    with pytest.raises(pkg.MfdUnsupportedError):
+        w.op.set_tile("fp64", 0, 1, 0)
+    with pytest.raises(pkg.MfdUnsupportedError):
+        w.op.h_from_e(np.zeros(w.n, dtype=complex))
+    w.close()
 
 
 def test_model2d_tm_is_five_point_laplacian(pkg):
