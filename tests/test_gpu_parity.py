@@ -1008,11 +1008,13 @@ def test_model2d_window_apply(kind, bloch, ty, kz, xo, pkg, O):
 
 
 @pytest.mark.parametrize("kind", ["te", "tm"])
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_model2d_window_solve(kind, precision, pkg, O):
-    """Two right-hand sides through the windowed 2D solve: each to 1e-9 in
-    the oracle's residual, against the direct solve, and against the solve of
-    the full single-cell 3D operator (three components per cell)."""
+@pytest.mark.parametrize("precision,method", [("fp64", None), ("mixed", None), ("fp64", "cocg"),
+                                              ("mixed", "cocg"), ("fp64", "bicgstab")])
+def test_model2d_window_solve(kind, precision, method, pkg, O):
+    """Two right-hand sides through the windowed 2D solve (SQMR, COCG, whose
+    step runs unfused with a window, and BiCGSTAB): each to 1e-9 in the
+    oracle's residual, against the direct solve, and against the solve of the
+    full single-cell 3D operator (three components per cell)."""
     T = pkg.twod
     om = 0.6
     m = model2d(pkg, kind, om=om)
@@ -1020,14 +1022,14 @@ def test_model2d_window_solve(kind, precision, pkg, O):
     rng = np.random.default_rng(43)
     B = rng.normal(size=(2, A2.shape[0])) + 1j * rng.normal(size=(2, A2.shape[0]))
     op = T.MaxwellOperator2D(m, om)
-    say("model2d window solve", kind, precision)
-    E, st = op.solve(B, tol=1e-9, precision=precision)
+    say("model2d window solve", kind, precision, method)
+    E, st = op.solve(B, tol=1e-9, precision=precision, method=method)
     op.close()
     full = pkg.MaxwellOperator(m.m3, om)
     sel = m.esel()
     B3 = np.zeros((2, full.n), dtype=complex)
     B3[:, sel] = B
-    E3, _ = full.solve(B3, tol=1e-9, precision=precision)
+    E3, _ = full.solve(B3, tol=1e-9, precision=precision, method=method)
     full.close()
     for r in range(2):
         assert np.linalg.norm(B[r] - A2 @ E[r]) / np.linalg.norm(B[r]) <= 1.05e-9
