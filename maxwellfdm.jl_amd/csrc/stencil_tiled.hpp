@@ -370,7 +370,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
                     } else xv = inw(c) ? rsg(c, cvt<double>(x[dof])) : czero<double>();
                     c64 qv = cone<double>();
                     if (MODE == MODE_P) {
-                        qv = od.qs[3 * p + c];   // q·T⁻¹
+                        if (inw(c)) qv = od.qs[3 * p + c];   // q·T⁻¹ (outside the window x is zero)
                         v = xv * qv;
                     } else v = xv;
                     c64 ev = cone<double>();
@@ -646,7 +646,8 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
                     int64_t cpy = j + 1 < g.N[1] ? p + g.N[0] : (g.bloch[1] ? p - (int64_t)(g.N[1] - 1) * g.N[0] : -1);
                     int64_t cpz = k + 1 < g.N[2] ? p + NxNy : (g.bloch[2] ? p - (int64_t)(g.N[2] - 1) * NxNy : -1);
                     // u = x: ww = y (xy) and ww = z (xz), corners p and p + x̂
-                    {
+                    // (a component outside the window is not written: no term)
+                    if (inw(0)) {
                         c64 s = (h5 * od.epso[3 * p + 0]) * (wy0 * Xs[bk][1][hj][hi] + wy1 * Xs[bk][1][hj - 1][hi]) +
                                 (h5 * od.epso[3 * p + 1]) * (wz0 * Xs[bk][2][hj][hi] + wz1 * zkm0);
                         if (cpx >= 0)
@@ -655,7 +656,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
                         term[0] = s;
                     }
                     // u = y: ww = x (xy) and ww = z (yz), corners p and p + ŷ
-                    {
+                    if (inw(1)) {
                         c64 s = (h5 * od.epso[3 * p + 0]) * (wx0 * Xs[bk][0][hj][hi] + wx1 * Xs[bk][0][hj][hi - 1]) +
                                 (h5 * od.epso[3 * p + 2]) * (wz0 * Xs[bk][2][hj][hi] + wz1 * zkm0);
                         if (cpy >= 0)
@@ -664,7 +665,7 @@ __global__ __launch_bounds__(TXW * TY, OFFD ? 2 : 1) void k_apply_tiled(DGrid<do
                         term[1] = s;
                     }
                     // u = z: ww = x (xz) and ww = y (yz), corners p and p + ẑ
-                    {
+                    if (inw(2)) {
                         c64 s = (h5 * od.epso[3 * p + 1]) * (wx0 * Xs[bk][0][hj][hi] + wx1 * Xs[bk][0][hj][hi - 1]) +
                                 (h5 * od.epso[3 * p + 2]) * (wy0 * Xs[bk][1][hj][hi] + wy1 * Xs[bk][1][hj - 1][hi]);
                         if (cpz >= 0) {   // plane k+1 (slab: plane k times the phase)
