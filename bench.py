@@ -62,6 +62,8 @@ def parse():
                     help="random-boxes: the metric line; the others time the batched-RHS, uneven-problem "
                          "and adjoint configurations")
     ap.add_argument("--nrhs", type=int, default=4, help="batched-rhs: right-hand sides")
+    ap.add_argument("--next-n", type=int, default=64,
+                    help="next-rows: edge of the random-box cube for the EE, HH and boundft = HH solves")
     ap.add_argument("--count", type=int, default=32, help="uneven-batch: problems")
     ap.add_argument("--voxels", type=int, default=16, help="adjoint: finite-difference voxels")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
@@ -634,8 +636,8 @@ def run_mixed_fgmres(a):
 
 
 def run_next_rows(a):
-    """The rows beyond the plain EE path, on the 64³ random-box model (FP64 to
-    1e-8): the EE solve (tiled stencil) next to the HH formulation and a
+    """The rows beyond the plain EE path, on the N³ random-box model (N =
+    --next-n, 64 by default; FP64 to 1e-8): the EE solve (tiled stencil) next to the HH formulation and a
     boundft = HH z axis (the tiled stencil on the reflected grid), h_from_e /
     e_from_h, create_Mcs, and a 1024² 2D TE solve (the single-z-cell
     operator restricted to Ex, Ey), with the default and the autotuned tile."""
@@ -644,7 +646,8 @@ def run_next_rows(a):
     from maxwellfdfd import twod as T
     from maxwellfdfd import workloads as W
     log = lambda *x: print("[next-rows]", *x, file=sys.stderr, flush=True)
-    out = {"workload": "next rows: 64^3 random boxes (EE, HH formulation, boundft HH on z), field conversion, "
+    N = a.next_n
+    out = {"workload": f"next rows: {N}^3 random boxes (EE, HH formulation, boundft HH on z), field conversion, "
                        "create_Mcs, 1024^2 2D TE; FP64 SQMR to 1e-8"}
 
     def solve(op, b):
@@ -653,7 +656,6 @@ def run_next_rows(a):
         return x, {"s": round(time.perf_counter() - t0, 3), "device_s": round(st.seconds, 3),
                    "iterations": int(st.iterations), "relres": st.relres}
 
-    N = 64
     m, om = W.random_boxes(N=N, npml=10, nbox=40, lam=24.0, seed=1, ndip=1)
     M.calc_matparams(m)
     je, jm = M.create_srcs(m)
