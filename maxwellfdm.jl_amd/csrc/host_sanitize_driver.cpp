@@ -73,6 +73,10 @@ int main() {
     const int rc = mfd_ctx_create(0, &ctx);   // without a GPU: an error code and a message
     if (rc == MFD_OK) mfd_ctx_destroy(ctx);
     else if (std::strlen(mfd_last_error()) == 0) return fail("no message for a failed context");
+    // operator entry points refuse a null handle before touching the device
+    if (mfd_op_set_components(nullptr, 0, 2) != MFD_ERR_ARG) return fail("set_components(null)");
+    if (mfd_op_set_tile(nullptr, MFD_FP64, 4, 16, 1) != MFD_ERR_ARG) return fail("set_tile(null)");
+    if (mfd_op_set_graddiv(nullptr, 1) != MFD_ERR_ARG) return fail("set_graddiv(null)");
     std::printf("host sanitize driver ok (%s)\n", mfd_version());
     return 0;
 }
