@@ -1038,6 +1038,34 @@ def test_model2d_window_solve(kind, precision, method, pkg, O):
         assert np.linalg.norm(E[r] - E3[r][sel]) / np.linalg.norm(xd) <= 1e-6
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_model2d_window_batch(precision, pkg, O):
+    """Windowed 2D operators of different sizes and polarisations (TE: two
+    unknowns per cell, TM: one) in one uneven-problem batch, against the
+    oracle's residual and the direct solve of each."""
+    T = pkg.twod
+    om = 0.6
+    cases = [("te", (14, 11)), ("tm", (9, 13)), ("te", (6, 7))]
+    ws, bs, As = [], [], []
+    for q, (kind, N) in enumerate(cases):
+        m = model2d(pkg, kind, N=N, om=om)
+        As.append(_oracle_2d(pkg, O, m, kind, om))
+        w = T.MaxwellOperator2D(m, om)
+        assert w.windowed
+        ws.append(w)
+        rng = np.random.default_rng(60 + q)
+        bs.append(rng.normal(size=w.n) + 1j * rng.normal(size=w.n))
+    say("model2d window batch", precision)
+    es, st = pkg.solve_batch([w.op for w in ws], [w._to_dev(b)[0] for w, b in zip(ws, bs)], tol=1e-9,
+                             precision=precision)
+    for w, A, b, e in zip(ws, As, bs, es):
+        x = w._from_dev(e, b.shape)
+        w.close()
+        assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) <= 1.05e-9
+        xd = sla.spsolve(A, b)
+        assert np.linalg.norm(x - xd) / np.linalg.norm(xd) <= 1e-6
+
+
 def test_model2d_window_fallback(pkg, O):
     """μ ≠ 1: the window is not built for it, so the 2D operator falls back
     to the full single-cell operator (three components per cell) and still
