@@ -241,13 +241,14 @@ int mfd_op_set_components(mfd_op *op, int c0, int nc);
  * Outputs may be null.  No reference counterpart. */
 int mfd_op_autotune(mfd_op *op, int64_t nrhs, int iters, int tile64[3], int tile32[3], double ms[2]);
 /* Fix the tile shape by hand: TY in {1,2,4,8} (0 = cache-reliant direct
- * kernel), KZ >= 1, xcd_order in {0,1,2,3} (bit 0: XCD tile order; bit 1:
- * 32-wide tiles of 2·TY rows, used by the diagonal-ε stencils and ignored by
- * the off-diagonal, grad-div and fused COCG ones).  Until this or mfd_op_autotune is
- * called, an operator uses TY = 4, KZ = 16, XCD order, with KZ and then TY
- * halved (down to 4 and 1) while a launch would have fewer than 256 tiles,
- * and 32-wide tiles (xcd_order 3) when ε is diagonal and they leave fewer
- * lanes of the last x tile idle than 64-wide ones. */
+ * kernel; MFD_ERR_UNSUPPORTED with a component window), KZ >= 1, xcd_order
+ * in {0,1,2,3} (bit 0: XCD tile order; bit 1: 32-wide tiles of 2·TY rows,
+ * used by the diagonal-ε, μ = 1 stencil of an EE-boundft operator and
+ * ignored by the others).  Until this or mfd_op_autotune is called, an
+ * operator uses TY = 4, KZ = 16, XCD order, with KZ and then TY halved (down
+ * to 4 and 1) while a launch would have fewer than 256 tiles, and 32-wide
+ * tiles (xcd_order 3) when that stencil serves and they leave fewer lanes of
+ * the last x tile idle than 64-wide ones. */
 int mfd_op_set_tile(mfd_op *op, int prec, int ty, int kz, int xcd_order);
 /* Average device time (ms) of one apply of the solver's stencil kernel
  * (MFD_FP64: A; MFD_MIXED: the complex-FP32 inner operator) */
