@@ -1066,6 +1066,36 @@ def test_model2d_window_batch(precision, pkg, O):
         assert np.linalg.norm(x - xd) / np.linalg.norm(xd) <= 1e-6
 
 
+@pytest.mark.parametrize("kind", ["te", "tm"])
+def test_model2d_window_noalloc(kind, pkg, O, devmem):
+    """A windowed 2D operator through mfd_solve_dev_noalloc (its workspace
+    sized for K unknowns per cell in the fixed scratch array) against
+    mfd_solve_dev: bitwise the same field and statistics, to 1e-9 in the
+    oracle's residual."""
+    T = pkg.twod
+    om = 0.6
+    m = model2d(pkg, kind, om=om)
+    A2 = _oracle_2d(pkg, O, m, kind, om)
+    w = T.MaxwellOperator2D(m, om)
+    rng = np.random.default_rng(71)
+    b = rng.normal(size=w.n) + 1j * rng.normal(size=w.n)
+    bd = w._to_dev(b)[0]
+    say("model2d window noalloc", kind)
+    out = []
+    for noalloc in (False, True):
+        db, de = devmem.put(bd), devmem.put(np.zeros_like(bd))
+        if noalloc:
+            st = w.op.solve_dev_noalloc(db, de, 1, tol=1e-9)
+        else:
+            st = w.op.solve_dev(db, de, 1, tol=1e-9, use_graph=False)
+        out.append((devmem.get(de, bd), st))
+    w.close()
+    (e0, s0), (e1, s1) = out
+    assert np.array_equal(e0, e1) and s0.iterations == s1.iterations and s0.relres == s1.relres
+    x = w._from_dev(e1, b.shape)
+    assert s1.converged and np.linalg.norm(b - A2 @ x) / np.linalg.norm(b) <= 1.05e-9
+
+
 def test_model2d_window_fallback(pkg, O):
     """μ ≠ 1: the window is not built for it, so the 2D operator falls back
     to the full single-cell operator (three components per cell) and still
