@@ -246,7 +246,8 @@ int mfd_op_autotune(mfd_op *op, int64_t nrhs, int iters, int tile64[3], int tile
  * used by the diagonal-ε, μ = 1 stencil of an EE-boundft operator and
  * ignored by the others).  Until this or mfd_op_autotune is called, an
  * operator uses TY = 4, KZ = 16, XCD order, with KZ and then TY halved (down
- * to 4 and 1) while a launch would have fewer than 256 tiles, and 32-wide
+ * to 4 and 1) while a launch would have fewer than 256 tiles, TY = 8 for
+ * full ε on a one-cell z axis where that still gives 256 tiles, and 32-wide
  * tiles (xcd_order 3) when that stencil serves and they leave fewer lanes of
  * the last x tile idle than 64-wide ones. */
 int mfd_op_set_tile(mfd_op *op, int prec, int ty, int kz, int xcd_order);

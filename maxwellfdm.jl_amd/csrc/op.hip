@@ -1381,6 +1381,10 @@ int op_create_impl(mfd_ctx *ctx, const mfd_grid *g, int ft, mfd_c64 omega, const
         int ty = 4, kz = 16;
         while (kz > 4 && tiles(ty, kz) < 256) kz /= 2;
         while (ty > 1 && tiles(ty, kz) < 256) ty /= 2;
+        // a one-cell z axis with full ε (the smoothed 2D models): each tile
+        // marches a single plane, and the 8-row tile is the faster one
+        // wherever it still gives a tile per CU (2D TE 1024²: 8.67 → 8.31 s)
+        if (full && op->G.h.N[2] == 1 && ty == 4 && tiles(8, kz) >= 256) ty = 8;
         op->tile64[0] = op->tile32[0] = ty;
         op->tile64[1] = op->tile32[1] = kz;
         op->tile64[2] = op->tile32[2] = narrow ? 3 : 1;
